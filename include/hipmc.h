@@ -1,0 +1,268 @@
+/*
+ * hipmc: C ABI of the MI355X-native tfp.mcmc gradient-based sampling path.
+ *
+ * Every entry point takes plain device pointers, sizes and a HIP stream
+ * (as uint64), returns 0 on success or a negative HIPMC_E* code, and never
+ * throws.  `hipmc_last_error()` returns a thread-local message for the last
+ * failure.  The library allocates no device memory: callers own every
+ * buffer (the Python layer passes torch / DLPack tensors).
+ *
+ * Reference interfaces replaced (paths relative to
+ * tensorflow_probability/python/):
+ *   hipmc_hmc_run        HamiltonianMonteCarlo.one_step  (mcmc/hmc.py:501-529)
+ *                        = MetropolisHastings.one_step (metropolis_hastings.py:160-254)
+ *                        over UncalibratedHamiltonianMonteCarlo.one_step
+ *                        (hmc.py:661-729); with n_iters > 1 also the
+ *                        sample_chain loop (mcmc/sample.py:311-383).
+ *   hipmc_nuts_run       NoUTurnSampler.one_step (mcmc/nuts.py:321-445) and
+ *                        its tree doubling (nuts.py:541-1010); n_iters > 1
+ *                        fuses the sample_chain loop.
+ *   hipmc_target_eval    target_log_prob_fn + autodiff gradient
+ *                        (mcmc/internal/util.py:246-308 ->
+ *                        math/gradient.py:28) for the target specs, with the
+ *                        TTK bijector composition
+ *                        (mcmc/transformed_kernel.py:86-128).
+ *   hipmc_da_update      DualAveragingStepSizeAdaptation._one_step_part
+ *                        (mcmc/dual_averaging_step_size_adaptation.py:353-475).
+ *   hipmc_mh_accept      MetropolisHastings accept/reject
+ *                        (mcmc/metropolis_hastings.py:204-246).
+ *   hipmc_leapfrog_run   UncalibratedHamiltonianMonteCarlo.one_step proposal
+ *                        (mcmc/hmc.py:661-729, leapfrog_integrator.py:222-362).
+ *   hipmc_bijector       Bijector.forward / inverse for TTK state mapping
+ *                        (bijectors/softplus.py:139-147, power_transform.py:85-100).
+ *   hipmc_draws          the momentum / uniform draws of samplers.normal /
+ *                        samplers.uniform (internal/samplers.py:308-367) under
+ *                        this library's counter-keyed Philox schedule.
+ */
+#ifndef HIPMC_H_
+#define HIPMC_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HIPMC_ABI_VERSION 1
+
+/* error codes */
+#define HIPMC_OK 0
+#define HIPMC_E_INVALID -1   /* bad argument (shape, dtype, null pointer) */
+#define HIPMC_E_UNSUPPORTED -2 /* valid but not built (e.g. dim too large) */
+#define HIPMC_E_HIP -3       /* a HIP runtime call failed */
+
+/* dtypes */
+#define HIPMC_F32 0
+#define HIPMC_F64 1
+
+/* target kinds */
+#define HIPMC_TARGET_LOGISTIC 1     /* beta ~ N(0,s^2), y ~ Bern(sigmoid(X beta)) */
+#define HIPMC_TARGET_LINEAR 2       /* [beta, sigma]: y ~ N(X beta, sigma), sigma ~ HalfNormal */
+#define HIPMC_TARGET_HIER_NORMAL 3  /* [mu, tau, z]: non-centred hierarchical normal */
+#define HIPMC_TARGET_DIAG_NORMAL 4  /* independent N(loc_i, scale_i) */
+#define HIPMC_TARGET_FUNNEL 5       /* Neal's funnel [v, x] */
+#define HIPMC_TARGET_NAN_EXP 6      /* sum(where(x<0, NaN, -x)) */
+
+/* bijector codes */
+#define HIPMC_BIJ_IDENTITY 0
+#define HIPMC_BIJ_SOFTPLUS 1
+#define HIPMC_BIJ_EXP 2
+
+#define HIPMC_MAX_BIJ_RANGES 4
+
+/*
+ * One posterior ("problem").  Lives in DEVICE memory as an array; the
+ * layout is fixed (offsets asserted in the library).  Data pointers are in
+ * the kernel dtype.
+ *   LOGISTIC / LINEAR: x = X [n_obs, dim_x] row-major, xt = X^T [dim_x, n_obs],
+ *                      y [n_obs]; param[0] = prior scale of beta,
+ *                      param[1] = HalfNormal scale of sigma (LINEAR).
+ *   HIER_NORMAL:       y [J], aux = sigma [J], n_obs = J;
+ *                      param[0] = mu prior scale, param[1] = tau scale.
+ *   DIAG_NORMAL:       y = loc [dim], aux = scale [dim].
+ *   FUNNEL / NAN_EXP:  no data.
+ * Bijector ranges [off, off+len) over the flat state, code per range.
+ */
+typedef struct hipmc_problem {
+  int32_t kind;
+  int32_t dim;      /* flat state size D (sum of part sizes) */
+  int32_t n_obs;
+  int32_t n_chains; /* chains of this problem */
+  int64_t chain_offset; /* first row of this problem in the state arrays */
+  const void* x;
+  const void* xt;
+  const void* y;
+  const void* aux;
+  double param[4];
+  double log_norm;  /* constant added to the log density (host-computed) */
+  int32_t n_bij;
+  int32_t bij_off[HIPMC_MAX_BIJ_RANGES];
+  int32_t bij_len[HIPMC_MAX_BIJ_RANGES];
+  int32_t bij_code[HIPMC_MAX_BIJ_RANGES];
+  int32_t pad_;
+} hipmc_problem;
+
+/*
+ * Arguments of the sampling kernels.  Rows of the state arrays are chains
+ * (stride `dim_stride` elements).  Chain c belongs to problem
+ * chain_problem[c] (NULL: problem 0) and is that problem's chain
+ * chain_local[c] (NULL: c); the pair keys its random draws, so results do
+ * not depend on which other chains share the launch.
+ *
+ * Seeds: if fold_seed != 0, iteration i uses fold_in(seed, iter0 + i);
+ * otherwise `seed` is the step seed itself (n_iters must be 1).
+ *
+ * Per-iteration outputs are [rows, n_chains] arrays; iteration i of the
+ * launch writes row out_row0 + i.  Any may be NULL.  trace_state, if set,
+ * is [n_trace, n_chains, dim_stride]: global iteration g = iter0 + i
+ * writes its post-step (unconstrained) state to row
+ * (g - trace_start) / trace_every when g >= trace_start, that difference is
+ * a multiple of trace_every and the row is < n_trace (sample_chain's
+ * burn-in and thinning).
+ */
+typedef struct hipmc_run_args {
+  int32_t dtype;
+  int32_t n_chains;
+  int32_t dim_stride;
+  int32_t max_dim;        /* max problem dim in the launch (<= dim_stride) */
+  int32_t n_iters;
+  int32_t fold_seed;
+  int64_t iter0;
+  uint32_t seed[2];
+  /* HMC */
+  int32_t num_leapfrog_steps;
+  /* NUTS */
+  int32_t max_tree_depth;
+  int32_t unrolled_leapfrog_steps;
+  /* kind of the launch's problems; LOGISTIC and LINEAR may be mixed */
+  int32_t target_kind;
+  double max_energy_diff;
+  /* problems and chain maps (device) */
+  const hipmc_problem* problems;
+  int32_t n_problems;
+  int32_t pad1_;
+  const int32_t* chain_problem;
+  const int32_t* chain_local;
+  /* step size (device, dtype): step_size[chain_step[c]] (chain_step NULL: 0) */
+  const void* step_size;
+  const int32_t* chain_step;
+  /* state in/out (device, dtype) */
+  void* state;   /* [n_chains, dim_stride] */
+  void* tlp;     /* [n_chains] */
+  void* grad;    /* [n_chains, dim_stride] */
+  /* per-iteration outputs (device), any may be NULL */
+  void* out_log_accept_ratio; /* dtype */
+  int32_t* out_leapfrogs;
+  uint8_t* out_is_accepted;
+  uint8_t* out_reach_max_depth;
+  uint8_t* out_has_divergence;
+  void* out_energy;           /* dtype; HMC: log_acceptance_correction */
+  int32_t* out_tree_depth;
+  int64_t out_row0;
+  void* trace_state;
+  int64_t trace_start;
+  int32_t trace_every;
+  int32_t n_trace;
+  /* HMC extras (last iteration only, any may be NULL), [n_chains, dim_stride] */
+  void* out_proposed_state;
+  void* out_proposed_tlp;     /* [n_chains] */
+  void* out_initial_momentum;
+  void* out_final_momentum;
+} hipmc_run_args;
+
+const char* hipmc_last_error(void);
+int hipmc_abi_version(void);
+
+/* Host-side seed derivation with the same Philox block the kernels use
+ * (role of internal/samplers.py:79-258 sanitize_seed / split_seed):
+ *   split:   out = philox(key=seed, ctr=(i, 0, 0, 'SPLT'))[0:2]
+ *   fold_in: out = philox(key=seed, ctr=(t_lo, t_hi, 0, 'FOLD'))[0:2]
+ *   salt:    out = philox(key=seed, ctr=(salt_hash, 0, 0, 'SALT'))[0:2] */
+int hipmc_seed_split(const uint32_t seed[2], uint32_t i, uint32_t out[2]);
+int hipmc_seed_fold_in(const uint32_t seed[2], uint64_t t, uint32_t out[2]);
+int hipmc_seed_salt(const uint32_t seed[2], uint32_t salt_hash,
+                    uint32_t out[2]);
+/* Philox4x32-10 block on the host (known-answer tests of the ABI). */
+int hipmc_philox_host(const uint32_t ctr[4], const uint32_t key[2],
+                      uint32_t out[4]);
+int hipmc_sizeof_problem(void);
+int hipmc_sizeof_run_args(void);
+
+/* Fused calibrated HMC (leapfrog + MH) for n_iters iterations. */
+int hipmc_hmc_run(const hipmc_run_args* args, uint64_t stream);
+
+/* Uncalibrated HMC proposal (n_iters must be 1): state/tlp/grad become the
+ * proposal; momenta and log_acceptance_correction (out_energy) written. */
+int hipmc_leapfrog_run(const hipmc_run_args* args, uint64_t stream);
+
+/* NUTS for n_iters iterations (per-chain wavefront state machine). */
+int hipmc_nuts_run(const hipmc_run_args* args, uint64_t stream);
+
+/* target log-prob and gradient at `x` [n_chains, dim_stride]. */
+int hipmc_target_eval(int32_t dtype, int32_t target_kind,
+                      const hipmc_problem* problems,
+                      const int32_t* chain_problem, int32_t n_chains,
+                      int32_t dim_stride, int32_t max_dim, const void* x,
+                      void* out_tlp, void* out_grad, uint64_t stream);
+
+/*
+ * Dual-averaging update of n_steps step-size elements, applied for n_rows
+ * successive DA steps (row r reads log_accept_ratio + r*row_stride and is DA
+ * step `step + r`).  Element s reads the
+ * chains of group g = step_group[s] (NULL: g = s), i.e. entries
+ * [group_start[g], group_start[g+1]) of `log_accept_ratio` (dtype; chains of
+ * a group contiguous).  DA state arrays are float64 [n_steps]: error_sum,
+ * log_averaging_step, log_shrinkage_target, target_accept_prob.  `step` is
+ * the DA step count before this update.  Writes step_size (dtype
+ * [n_steps]) and, if non-NULL, new_step_size_f64 (which also supplies the
+ * previous step size when set).
+ */
+int hipmc_da_update(int32_t dtype, int32_t n_steps, const int32_t* step_group,
+                    const int32_t* group_start, const void* log_accept_ratio,
+                    int32_t n_rows, int64_t row_stride,
+                    double* error_sum, double* log_averaging_step,
+                    const double* log_shrinkage_target,
+                    const double* target_accept_prob, int64_t step,
+                    int64_t num_adaptation_steps, double exploration_shrinkage,
+                    double step_count_smoothing, double decay_rate,
+                    void* step_size, double* new_step_size_f64,
+                    uint64_t stream);
+
+/*
+ * Metropolis-Hastings accept for n_chains chains:
+ *   lar = safe_sum(proposed_tlp - current_tlp [+ correction]);
+ *   accept = log(U) < lar, U from the MH purpose keyed by `seed`.
+ * Rows of `current` (state [n, dim_stride], tlp, grad) are overwritten by the
+ * proposal where accepted.  correction may be NULL.
+ */
+int hipmc_mh_accept(int32_t dtype, int32_t n_chains, int32_t dim_stride,
+                    const uint32_t seed[2], const int32_t* chain_problem,
+                    const int32_t* chain_local, const void* proposed_state,
+                    const void* proposed_tlp, const void* proposed_grad,
+                    const void* correction, void* state, void* tlp,
+                    void* grad, void* out_log_accept_ratio,
+                    uint8_t* out_is_accepted, uint64_t stream);
+
+/* Elementwise bijector on x [n_chains, dim_stride] per the problem's bijector
+ * ranges: direction 0 = forward, 1 = inverse. */
+int hipmc_bijector(int32_t dtype, const hipmc_problem* problems,
+                   const int32_t* chain_problem, int32_t n_chains,
+                   int32_t dim_stride, int32_t direction, const void* x,
+                   void* out, uint64_t stream);
+
+/*
+ * Raw draws of the schedule, for tests and for callers that need the same
+ * stream on the host side.  kind 0: momentum [n_chains, dim];
+ * kind 1: MH uniform [n_chains]; kind 2: NUTS tree draws for `depth`
+ * ([n_chains] direction as 0/1 in out_aux (uint8) and uniform in out);
+ * kind 3: NUTS leaf uniform for (`depth`, `index`) [n_chains].
+ */
+int hipmc_draws(int32_t dtype, int32_t kind, const uint32_t seed[2],
+                int32_t problem, int32_t n_chains, int32_t dim, int32_t depth,
+                int32_t index, void* out, uint8_t* out_aux, uint64_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HIPMC_H_ */

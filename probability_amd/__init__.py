@@ -1,0 +1,14 @@
+"""probability_amd: MI355X-native tfp.mcmc gradient-based sampling.
+
+`probability_amd.mcmc` mirrors `tfp.mcmc` (HamiltonianMonteCarlo,
+NoUTurnSampler, TransformedTransitionKernel,
+DualAveragingStepSizeAdaptation, sample_chain, diagnostics) on top of
+libhipmc, the HIP kernels behind the C ABI in include/hipmc.h.
+"""
+
+from probability_amd import bijectors
+from probability_amd import mcmc
+from probability_amd import random
+from probability_amd import targets
+
+__version__ = '0.1.0'

@@ -1,0 +1,545 @@
+// C ABI of libhipmc: argument checks, launch configuration and the small
+// auxiliary kernels (dual averaging, MH accept, bijectors, raw draws).
+// No C++ exception crosses this boundary; errors are negative return codes
+// plus a thread-local message.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstddef>
+#include <cstdio>
+#include <string>
+
+#include "../../include/hipmc.h"
+#include "common.h"
+#include "kernels.h"
+#include "targets.h"
+
+using namespace hipmc;
+
+static_assert(sizeof(hipmc_problem) == 152, "hipmc_problem layout");
+static_assert(offsetof(hipmc_problem, param) == 56, "hipmc_problem layout");
+static_assert(offsetof(hipmc_problem, n_bij) == 96, "hipmc_problem layout");
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+int hip_check(hipError_t e, const char* what) {
+  if (e == hipSuccess) return HIPMC_OK;
+  return fail(HIPMC_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+hipStream_t as_stream(uint64_t s) { return reinterpret_cast<hipStream_t>(s); }
+
+constexpr int kMaxLdsPerBlock = 64 * 1024;
+
+// Smallest compiled slot count K with K*64 >= max_dim.
+int pick_k(int max_dim) {
+  static const int ks[] = {1, 2, 3, 4, 5, 8};
+  for (int k : ks)
+    if (k * 64 >= max_dim) return k;
+  return -1;
+}
+
+int waves_for_lds(size_t per_wave_bytes) {
+  int w = 4;
+  while (w > 1 && (size_t)w * per_wave_bytes > (size_t)kMaxLdsPerBlock) --w;
+  return w;
+}
+
+int check_run(const hipmc_run_args* a, bool nuts) {
+  if (!a) return fail(HIPMC_E_INVALID, "args is NULL");
+  if (a->dtype != HIPMC_F32 && a->dtype != HIPMC_F64)
+    return fail(HIPMC_E_INVALID, "dtype must be HIPMC_F32 or HIPMC_F64");
+  if (a->n_chains < 0) return fail(HIPMC_E_INVALID, "n_chains < 0");
+  if (a->n_iters < 1) return fail(HIPMC_E_INVALID, "n_iters must be >= 1");
+  if (!a->fold_seed && a->n_iters != 1)
+    return fail(HIPMC_E_INVALID, "n_iters > 1 requires fold_seed");
+  if (a->max_dim < 1 || a->max_dim > a->dim_stride)
+    return fail(HIPMC_E_INVALID, "need 1 <= max_dim <= dim_stride");
+  if (pick_k(a->max_dim) < 0)
+    return fail(HIPMC_E_UNSUPPORTED, "state dimension above 512 is not built");
+  if (!a->problems || a->n_problems < 1)
+    return fail(HIPMC_E_INVALID, "problems is NULL");
+  if (!a->step_size || !a->state || !a->tlp || !a->grad)
+    return fail(HIPMC_E_INVALID, "step_size/state/tlp/grad must be set");
+  if (a->trace_state && (a->trace_every < 1 || a->n_trace < 0))
+    return fail(HIPMC_E_INVALID, "trace_every must be >= 1 and n_trace >= 0");
+  if (a->out_row0 < 0) return fail(HIPMC_E_INVALID, "out_row0 < 0");
+  if (nuts) {
+    if (a->max_tree_depth < 1 || a->max_tree_depth > 16)
+      return fail(HIPMC_E_INVALID, "max_tree_depth must be in [1, 16]");
+    if (a->unrolled_leapfrog_steps < 1)
+      return fail(HIPMC_E_INVALID, "unrolled_leapfrog_steps must be >= 1");
+  } else if (a->num_leapfrog_steps < 0) {
+    return fail(HIPMC_E_INVALID, "num_leapfrog_steps must be >= 0");
+  }
+  return HIPMC_OK;
+}
+
+template <typename T, int K, int FAM>
+int launch_nuts(const hipmc_run_args& a, hipStream_t s) {
+  const size_t per_wave = nuts_lds_elems<T, K>(a.max_tree_depth) * sizeof(T);
+  if (per_wave > (size_t)kMaxLdsPerBlock)
+    return fail(HIPMC_E_UNSUPPORTED, "U-turn memory does not fit in LDS");
+  const int w = waves_for_lds(per_wave);
+  const int blocks = (a.n_chains + w - 1) / w;
+  if (blocks == 0) return HIPMC_OK;
+  hipLaunchKernelGGL((nuts_kernel<T, K, FAM>), dim3(blocks), dim3(64 * w),
+                     per_wave * w, s, a);
+  return hip_check(hipGetLastError(), "nuts_kernel launch");
+}
+
+template <typename T, int K, int FAM, bool CAL>
+int launch_hmc(const hipmc_run_args& a, hipStream_t s) {
+  const size_t per_wave = hmc_lds_elems<T, K>() * sizeof(T);
+  const int w = 4;
+  const int blocks = (a.n_chains + w - 1) / w;
+  if (blocks == 0) return HIPMC_OK;
+  hipLaunchKernelGGL((hmc_kernel<T, K, FAM, CAL>), dim3(blocks), dim3(64 * w),
+                     per_wave * w, s, a);
+  return hip_check(hipGetLastError(), "hmc_kernel launch");
+}
+
+template <typename T, int FAM, template <typename, int, int> class F,
+          typename... A>
+int dispatch_k(int k, A&&... args) {
+  switch (k) {
+    case 1: return F<T, 1, FAM>::run(args...);
+    case 2: return F<T, 2, FAM>::run(args...);
+    case 3: return F<T, 3, FAM>::run(args...);
+    case 4: return F<T, 4, FAM>::run(args...);
+    case 5: return F<T, 5, FAM>::run(args...);
+    case 8: return F<T, 8, FAM>::run(args...);
+  }
+  return fail(HIPMC_E_UNSUPPORTED, "no kernel for this dimension");
+}
+
+template <template <typename, int, int> class F, typename... A>
+int dispatch(int dtype, int kind, int k, A&&... args) {
+  if (kind < HIPMC_TARGET_LOGISTIC || kind > HIPMC_TARGET_NAN_EXP)
+    return fail(HIPMC_E_INVALID, "unknown target kind");
+  const int fam = family_of(kind);
+  if (dtype == HIPMC_F32) {
+    if (fam == kFamilyHier) return dispatch_k<float, kFamilyHier, F>(k, args...);
+    if (fam == kFamilyGlm) return dispatch_k<float, kFamilyGlm, F>(k, args...);
+    return dispatch_k<float, kFamilyTest, F>(k, args...);
+  }
+  if (dtype == HIPMC_F64) {
+    if (fam == kFamilyHier) return dispatch_k<double, kFamilyHier, F>(k, args...);
+    if (fam == kFamilyGlm) return dispatch_k<double, kFamilyGlm, F>(k, args...);
+    return dispatch_k<double, kFamilyTest, F>(k, args...);
+  }
+  return fail(HIPMC_E_INVALID, "bad dtype");
+}
+
+template <typename T, int K, int FAM>
+struct NutsL {
+  static int run(const hipmc_run_args& a, hipStream_t s) {
+    return launch_nuts<T, K, FAM>(a, s);
+  }
+};
+template <typename T, int K, int FAM>
+struct HmcL {
+  static int run(const hipmc_run_args& a, hipStream_t s) {
+    return launch_hmc<T, K, FAM, true>(a, s);
+  }
+};
+template <typename T, int K, int FAM>
+struct LeapL {
+  static int run(const hipmc_run_args& a, hipStream_t s) {
+    return launch_hmc<T, K, FAM, false>(a, s);
+  }
+};
+template <typename T, int K, int FAM>
+struct EvalL {
+  static int run(const hipmc_problem* p, const int32_t* cp, int n, int stride,
+                 const void* x, void* tlp, void* grad, hipStream_t s) {
+    const size_t per_wave = hmc_lds_elems<T, K>() * sizeof(T);
+    const int w = 4;
+    const int blocks = (n + w - 1) / w;
+    if (blocks == 0) return HIPMC_OK;
+    hipLaunchKernelGGL((eval_kernel<T, K, FAM>), dim3(blocks), dim3(64 * w),
+                       per_wave * w, s, p, cp, n, stride, (const T*)x,
+                       (T*)tlp, (T*)grad);
+    return hip_check(hipGetLastError(), "eval_kernel launch");
+  }
+};
+
+// ------------------------------------------------------------ DA kernel
+// One block per step-size group; float64 throughout
+// (dual_averaging_step_size_adaptation.py:399-475).
+template <typename T>
+__global__ void __launch_bounds__(256)
+    da_kernel(const int32_t* step_group, const int32_t* group_start,
+              const T* lar_base, int n_rows, int64_t row_stride,
+              double* error_sum, double* log_avg, const double* log_shrink,
+              const double* target_accept, int64_t step0, int64_t num_adapt,
+              double gamma, double t0, double kappa, T* step_size,
+              double* step_f64) {
+  __shared__ double red[256];
+  const int gidx = blockIdx.x;
+  const int grp = step_group ? step_group[gidx] : gidx;
+  const int b = group_start[grp], e = group_start[grp + 1];
+  const int tid = threadIdx.x;
+  double err = error_sum[gidx];
+  double la = log_avg[gidx];
+  double cur = step_f64 ? step_f64[gidx] : (double)step_size[gidx];
+  for (int r = 0; r < n_rows; ++r) {
+    const T* lar = lar_base + (size_t)r * row_stride;
+    auto lap_of = [&](int i) -> double {
+      double v = (double)lar[i];
+      if (!__builtin_isfinite(v)) v = -__builtin_inf();
+      return fmin(v, 0.0);  // hmc_like_log_accept_prob_getter_fn
+    };
+    double m = -__builtin_inf();
+    for (int i = b + tid; i < e; i += blockDim.x) m = fmax(m, lap_of(i));
+    red[tid] = m;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (tid < s) red[tid] = fmax(red[tid], red[tid + s]);
+      __syncthreads();
+    }
+    double mx = red[0];
+    if (!__builtin_isfinite(mx)) mx = 0.0;
+    __syncthreads();
+    double acc = 0.0;
+    for (int i = b + tid; i < e; i += blockDim.x) acc += exp(lap_of(i) - mx);
+    red[tid] = acc;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+      if (tid < s) red[tid] += red[tid + s];
+      __syncthreads();
+    }
+    const double total = red[0];
+    __syncthreads();
+    // reduce_logmeanexp, then `_one_step_part`
+    const double rlap = log(total) + mx - log((double)(e - b));
+    err = err + target_accept[gidx] - exp(rlap);
+    const int64_t step = step0 + r;
+    const double t = (double)step + 1.0;
+    const double new_log_step =
+        log_shrink[gidx] - err * sqrt(t) / ((t0 + t) * gamma);
+    const double eta = pow(t, -kappa);
+    const double nla = eta * new_log_step + (1.0 - eta) * la;
+    const int64_t st = step + 1;
+    if (st < num_adapt) cur = exp(new_log_step);
+    else if (st == num_adapt) cur = exp(nla);
+    if (st <= num_adapt) la = nla;
+  }
+  if (tid == 0) {
+    error_sum[gidx] = err;
+    log_avg[gidx] = la;
+    step_size[gidx] = (T)cur;
+    if (step_f64) step_f64[gidx] = cur;
+  }
+}
+
+// ------------------------------------------------------------ MH kernels
+template <typename T>
+__global__ void mh_decide_kernel(int n, Seed sd, const int32_t* chain_problem,
+                                 const int32_t* chain_local, const T* ptlp,
+                                 const T* corr, T* tlp, T* out_lar,
+                                 uint8_t* accepted) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  const uint32_t prob = chain_problem ? chain_problem[c] : 0;
+  const uint32_t local = chain_local ? chain_local[c] : c;
+  double lar = (double)ptlp[c] - (double)tlp[c];
+  if (corr) lar += (double)corr[c];
+  if (!__builtin_isfinite(lar)) lar = -__builtin_inf();
+  const U4 w = philox(U4{0u, local, prob, kPurposeMHAccept}, sd.k0, sd.k1);
+  const double u = (double)uniform_from<T>(w.x, w.y);
+  const bool acc = log(u) < lar;
+  accepted[c] = acc;
+  if (out_lar) out_lar[c] = (T)lar;
+  if (acc) tlp[c] = ptlp[c];
+}
+
+template <typename T>
+__global__ void select_rows_kernel(int n, int stride, const uint8_t* accepted,
+                                   const T* ps, const T* pg, T* s, T* g) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)n * stride) return;
+  if (accepted[i / stride]) {
+    s[i] = ps[i];
+    g[i] = pg[i];
+  }
+}
+
+// ------------------------------------------------------------ bijectors
+template <typename T>
+__device__ inline T softplus_inverse(T y) {
+  // math/generic.py:571-584
+  const T eps = sizeof(T) == 4 ? (T)1.1920928955078125e-07
+                               : (T)2.220446049250313e-16;
+  const T threshold = log(eps) + T(2);
+  const bool too_small = y < exp(threshold);
+  const bool too_large = y > -threshold;
+  const T safe = (too_small || too_large) ? T(1) : y;
+  const T mid = safe + log(-expm1(-safe));
+  return too_small ? log(y) : (too_large ? y : mid);
+}
+
+template <typename T>
+__global__ void bijector_kernel(const hipmc_problem* problems,
+                                const int32_t* chain_problem, int n,
+                                int stride, int direction, const T* x, T* out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (size_t)n * stride) return;
+  const int c = (int)(i / stride);
+  const int e = (int)(i % stride);
+  const hipmc_problem& pb = problems[chain_problem ? chain_problem[c] : 0];
+  T v = x[i];
+  int code = HIPMC_BIJ_IDENTITY;
+  for (int r = 0; r < pb.n_bij; ++r)
+    if (e >= pb.bij_off[r] && e < pb.bij_off[r] + pb.bij_len[r])
+      code = pb.bij_code[r];
+  if (code == HIPMC_BIJ_SOFTPLUS)
+    v = direction == 0 ? softplus_forward(v) : softplus_inverse(v);
+  else if (code == HIPMC_BIJ_EXP)
+    v = direction == 0 ? exp(v) : log(v);
+  out[i] = v;
+}
+
+// ------------------------------------------------------------ raw draws
+template <typename T>
+__global__ void draws_kernel(int kind, Seed sd, uint32_t prob, int n, int dim,
+                             int depth, int index, T* out, uint8_t* aux) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (kind == 0) {
+    if (i >= (size_t)n * dim) return;
+    const uint32_t c = (uint32_t)(i / dim);
+    const int e = (int)(i % dim);
+    const U4 w = philox(U4{(uint32_t)(e >> 2), c, prob, kPurposeMomentum},
+                        sd.k0, sd.k1);
+    const bool hi = (e & 2) != 0;
+    double n0, n1;
+    box_muller(hi ? w.z : w.x, hi ? w.w : w.y, n0, n1);
+    out[i] = (T)((e & 1) ? n1 : n0);
+    return;
+  }
+  if (i >= (size_t)n) return;
+  const uint32_t c = (uint32_t)i;
+  if (kind == 1) {
+    const U4 w = philox(U4{0u, c, prob, kPurposeMHAccept}, sd.k0, sd.k1);
+    out[i] = uniform_from<T>(w.x, w.y);
+  } else if (kind == 2) {
+    const U4 w = philox(U4{0u, c, prob, kPurposeTree | (uint32_t)depth},
+                        sd.k0, sd.k1);
+    if (aux) aux[i] = (uint8_t)(w.x >> 31);
+    out[i] = uniform_from<T>(w.y, w.z);
+  } else {
+    const U4 w = philox(U4{(uint32_t)index, c, prob,
+                           kPurposeLeaf | (uint32_t)depth},
+                        sd.k0, sd.k1);
+    out[i] = uniform_from<T>(w.x, w.y);
+  }
+}
+
+int blocks_for(size_t n, int bs) { return (int)((n + bs - 1) / bs); }
+
+}  // namespace
+
+extern "C" {
+
+const char* hipmc_last_error(void) { return g_err.c_str(); }
+int hipmc_abi_version(void) { return HIPMC_ABI_VERSION; }
+int hipmc_sizeof_problem(void) { return (int)sizeof(hipmc_problem); }
+
+int hipmc_seed_split(const uint32_t seed[2], uint32_t i, uint32_t out[2]) {
+  if (!seed || !out) return fail(HIPMC_E_INVALID, "seed_split: NULL argument");
+  const Seed r = split(Seed{seed[0], seed[1]}, i);
+  out[0] = r.k0;
+  out[1] = r.k1;
+  return HIPMC_OK;
+}
+
+int hipmc_seed_fold_in(const uint32_t seed[2], uint64_t t, uint32_t out[2]) {
+  if (!seed || !out) return fail(HIPMC_E_INVALID, "seed_fold_in: NULL argument");
+  const Seed r = fold_in(Seed{seed[0], seed[1]}, t);
+  out[0] = r.k0;
+  out[1] = r.k1;
+  return HIPMC_OK;
+}
+
+int hipmc_seed_salt(const uint32_t seed[2], uint32_t salt_hash,
+                    uint32_t out[2]) {
+  if (!seed || !out) return fail(HIPMC_E_INVALID, "seed_salt: NULL argument");
+  const U4 o = philox(U4{salt_hash, 0u, 0u, 0x53414C54u}, seed[0], seed[1]);
+  out[0] = o.x;
+  out[1] = o.y;
+  return HIPMC_OK;
+}
+
+int hipmc_philox_host(const uint32_t ctr[4], const uint32_t key[2],
+                      uint32_t out[4]) {
+  if (!ctr || !key || !out) return fail(HIPMC_E_INVALID, "philox_host: NULL argument");
+  const U4 o = philox(U4{ctr[0], ctr[1], ctr[2], ctr[3]}, key[0], key[1]);
+  out[0] = o.x;
+  out[1] = o.y;
+  out[2] = o.z;
+  out[3] = o.w;
+  return HIPMC_OK;
+}
+int hipmc_sizeof_run_args(void) { return (int)sizeof(hipmc_run_args); }
+
+int hipmc_nuts_run(const hipmc_run_args* a, uint64_t stream) {
+  int rc = check_run(a, true);
+  if (rc) return rc;
+  return dispatch<NutsL>(a->dtype, a->target_kind, pick_k(a->max_dim), *a,
+                         as_stream(stream));
+}
+
+int hipmc_hmc_run(const hipmc_run_args* a, uint64_t stream) {
+  int rc = check_run(a, false);
+  if (rc) return rc;
+  return dispatch<HmcL>(a->dtype, a->target_kind, pick_k(a->max_dim), *a,
+                        as_stream(stream));
+}
+
+int hipmc_leapfrog_run(const hipmc_run_args* a, uint64_t stream) {
+  int rc = check_run(a, false);
+  if (rc) return rc;
+  if (a->n_iters != 1) return fail(HIPMC_E_INVALID, "leapfrog_run: n_iters must be 1");
+  return dispatch<LeapL>(a->dtype, a->target_kind, pick_k(a->max_dim), *a,
+                         as_stream(stream));
+}
+
+int hipmc_target_eval(int32_t dtype, int32_t target_kind,
+                      const hipmc_problem* problems,
+                      const int32_t* chain_problem, int32_t n_chains,
+                      int32_t dim_stride, int32_t max_dim, const void* x,
+                      void* out_tlp, void* out_grad, uint64_t stream) {
+  if (!problems || !x) return fail(HIPMC_E_INVALID, "problems/x is NULL");
+  if (max_dim < 1 || max_dim > dim_stride)
+    return fail(HIPMC_E_INVALID, "need 1 <= max_dim <= dim_stride");
+  const int k = pick_k(max_dim);
+  if (k < 0) return fail(HIPMC_E_UNSUPPORTED, "state dimension above 512 is not built");
+  return dispatch<EvalL>(dtype, target_kind, k, problems, chain_problem,
+                         n_chains, dim_stride, x, out_tlp, out_grad,
+                         as_stream(stream));
+}
+
+int hipmc_da_update(int32_t dtype, int32_t n_groups, const int32_t* step_group,
+                    const int32_t* group_start, const void* lar,
+                    int32_t n_rows, int64_t row_stride,
+                    double* error_sum, double* log_averaging_step,
+                    const double* log_shrinkage_target,
+                    const double* target_accept_prob, int64_t step,
+                    int64_t num_adaptation_steps, double exploration_shrinkage,
+                    double step_count_smoothing, double decay_rate,
+                    void* step_size, double* new_step_size_f64,
+                    uint64_t stream) {
+  if (n_groups < 1 || !group_start || !lar || !error_sum ||
+      !log_averaging_step || !log_shrinkage_target || !target_accept_prob ||
+      !step_size)
+    return fail(HIPMC_E_INVALID, "da_update: NULL argument or n_groups < 1");
+  if (n_rows < 1) return fail(HIPMC_E_INVALID, "da_update: n_rows must be >= 1");
+  hipStream_t s = as_stream(stream);
+  if (dtype == HIPMC_F32)
+    hipLaunchKernelGGL(da_kernel<float>, dim3(n_groups), dim3(256), 0, s,
+                       step_group, group_start, (const float*)lar, n_rows,
+                       row_stride, error_sum,
+                       log_averaging_step, log_shrinkage_target,
+                       target_accept_prob, step, num_adaptation_steps,
+                       exploration_shrinkage, step_count_smoothing, decay_rate,
+                       (float*)step_size, new_step_size_f64);
+  else if (dtype == HIPMC_F64)
+    hipLaunchKernelGGL(da_kernel<double>, dim3(n_groups), dim3(256), 0, s,
+                       step_group, group_start, (const double*)lar, n_rows,
+                       row_stride, error_sum,
+                       log_averaging_step, log_shrinkage_target,
+                       target_accept_prob, step, num_adaptation_steps,
+                       exploration_shrinkage, step_count_smoothing, decay_rate,
+                       (double*)step_size, new_step_size_f64);
+  else
+    return fail(HIPMC_E_INVALID, "bad dtype");
+  return hip_check(hipGetLastError(), "da_kernel launch");
+}
+
+int hipmc_mh_accept(int32_t dtype, int32_t n, int32_t stride,
+                    const uint32_t seed[2], const int32_t* chain_problem,
+                    const int32_t* chain_local, const void* ps, const void* pt,
+                    const void* pg, const void* corr, void* state, void* tlp,
+                    void* grad, void* out_lar, uint8_t* out_acc,
+                    uint64_t stream) {
+  if (!seed || !ps || !pt || !pg || !state || !tlp || !grad || !out_acc)
+    return fail(HIPMC_E_INVALID, "mh_accept: NULL argument");
+  if (n <= 0) return HIPMC_OK;
+  hipStream_t s = as_stream(stream);
+  const Seed sd{seed[0], seed[1]};
+  const size_t ne = (size_t)n * stride;
+  if (dtype == HIPMC_F32) {
+    hipLaunchKernelGGL(mh_decide_kernel<float>, dim3(blocks_for(n, 256)),
+                       dim3(256), 0, s, n, sd, chain_problem, chain_local,
+                       (const float*)pt, (const float*)corr, (float*)tlp,
+                       (float*)out_lar, out_acc);
+    hipLaunchKernelGGL(select_rows_kernel<float>, dim3(blocks_for(ne, 256)),
+                       dim3(256), 0, s, n, stride, out_acc, (const float*)ps,
+                       (const float*)pg, (float*)state, (float*)grad);
+  } else if (dtype == HIPMC_F64) {
+    hipLaunchKernelGGL(mh_decide_kernel<double>, dim3(blocks_for(n, 256)),
+                       dim3(256), 0, s, n, sd, chain_problem, chain_local,
+                       (const double*)pt, (const double*)corr, (double*)tlp,
+                       (double*)out_lar, out_acc);
+    hipLaunchKernelGGL(select_rows_kernel<double>, dim3(blocks_for(ne, 256)),
+                       dim3(256), 0, s, n, stride, out_acc, (const double*)ps,
+                       (const double*)pg, (double*)state, (double*)grad);
+  } else {
+    return fail(HIPMC_E_INVALID, "bad dtype");
+  }
+  return hip_check(hipGetLastError(), "mh kernels launch");
+}
+
+int hipmc_bijector(int32_t dtype, const hipmc_problem* problems,
+                   const int32_t* chain_problem, int32_t n, int32_t stride,
+                   int32_t direction, const void* x, void* out,
+                   uint64_t stream) {
+  if (!problems || !x || !out) return fail(HIPMC_E_INVALID, "bijector: NULL argument");
+  if (direction != 0 && direction != 1)
+    return fail(HIPMC_E_INVALID, "bijector: direction must be 0 or 1");
+  const size_t ne = (size_t)n * stride;
+  if (ne == 0) return HIPMC_OK;
+  hipStream_t s = as_stream(stream);
+  if (dtype == HIPMC_F32)
+    hipLaunchKernelGGL(bijector_kernel<float>, dim3(blocks_for(ne, 256)),
+                       dim3(256), 0, s, problems, chain_problem, n, stride,
+                       direction, (const float*)x, (float*)out);
+  else if (dtype == HIPMC_F64)
+    hipLaunchKernelGGL(bijector_kernel<double>, dim3(blocks_for(ne, 256)),
+                       dim3(256), 0, s, problems, chain_problem, n, stride,
+                       direction, (const double*)x, (double*)out);
+  else
+    return fail(HIPMC_E_INVALID, "bad dtype");
+  return hip_check(hipGetLastError(), "bijector_kernel launch");
+}
+
+int hipmc_draws(int32_t dtype, int32_t kind, const uint32_t seed[2],
+                int32_t problem, int32_t n, int32_t dim, int32_t depth,
+                int32_t index, void* out, uint8_t* aux, uint64_t stream) {
+  if (!seed || !out) return fail(HIPMC_E_INVALID, "draws: NULL argument");
+  if (kind < 0 || kind > 3) return fail(HIPMC_E_INVALID, "draws: bad kind");
+  const size_t ne = kind == 0 ? (size_t)n * dim : (size_t)n;
+  if (ne == 0) return HIPMC_OK;
+  hipStream_t s = as_stream(stream);
+  const Seed sd{seed[0], seed[1]};
+  if (dtype == HIPMC_F32)
+    hipLaunchKernelGGL(draws_kernel<float>, dim3(blocks_for(ne, 256)),
+                       dim3(256), 0, s, kind, sd, (uint32_t)problem, n, dim,
+                       depth, index, (float*)out, aux);
+  else if (dtype == HIPMC_F64)
+    hipLaunchKernelGGL(draws_kernel<double>, dim3(blocks_for(ne, 256)),
+                       dim3(256), 0, s, kind, sd, (uint32_t)problem, n, dim,
+                       depth, index, (double*)out, aux);
+  else
+    return fail(HIPMC_E_INVALID, "bad dtype");
+  return hip_check(hipGetLastError(), "draws_kernel launch");
+}
+
+}  // extern "C"

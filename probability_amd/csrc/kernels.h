@@ -1,0 +1,471 @@
+// Sampling kernels: one wavefront = one chain.
+//
+// * nuts_kernel: NoUTurnSampler.one_step (tensorflow_probability/python/
+//   mcmc/nuts.py:321-1010) as a per-chain state machine.  A chain's control
+//   flow is wave-uniform, so a chain that stops (U-turn, divergence) leaves
+//   its loop instead of idling under a mask until the slowest chain of the
+//   batch is done, which is what the lockstep tf.while_loop does
+//   (nuts.py:404-422, 756-759).  Outputs are identical because TFP freezes a
+//   stopped chain's outputs and every draw here is keyed per chain.
+// * hmc_kernel: MetropolisHastings(UncalibratedHamiltonianMonteCarlo)
+//   (mcmc/hmc.py:661-875, mcmc/metropolis_hastings.py:160-254) fused with
+//   the sample_chain loop over n_iters iterations.
+//
+// State element e of a chain lives in lane e % 64, register slot e / 64;
+// the per-wave LDS block holds the NUTS U-turn memory and target scratch.
+#pragma once
+
+#include "../../include/hipmc.h"
+#include "common.h"
+#include "targets.h"
+
+namespace hipmc {
+
+template <typename T>
+struct Run {
+  const hipmc_run_args& a;
+  __device__ const T* step_size() const { return (const T*)a.step_size; }
+};
+
+struct ChainId {
+  int chain;   // row in the launch
+  int prob;    // problem index
+  int local;   // chain index inside its problem (keys the draws)
+};
+
+__device__ inline ChainId chain_of(const hipmc_run_args& a, int c) {
+  ChainId id;
+  id.chain = c;
+  id.prob = a.chain_problem ? a.chain_problem[c] : 0;
+  id.local = a.chain_local ? a.chain_local[c] : c;
+  return id;
+}
+
+__device__ inline Seed iter_seed(const hipmc_run_args& a, int it) {
+  Seed base{a.seed[0], a.seed[1]};
+  return a.fold_seed ? fold_in(base, (uint64_t)(a.iter0 + it)) : base;
+}
+
+// Standard-normal momentum of the chain: element e uses Philox block e/4,
+// word pair (e%4)/2, cos for even e, sin for odd e.
+template <typename T, int K>
+__device__ inline void draw_momentum(Seed s, const ChainId& id, int dim,
+                                     T (&p)[K]) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int e = k * 64 + lane;
+    const U4 w = philox(U4{(uint32_t)(e >> 2), (uint32_t)id.local,
+                           (uint32_t)id.prob, kPurposeMomentum},
+                        s.k0, s.k1);
+    const bool hi_pair = (e & 2) != 0;
+    const uint32_t wa = hi_pair ? w.z : w.x;
+    const uint32_t wb = hi_pair ? w.w : w.y;
+    double n0, n1;
+    box_muller(wa, wb, n0, n1);
+    const double n = (e & 1) ? n1 : n0;
+    p[k] = (e < dim) ? (T)n : T(0);
+  }
+}
+
+template <typename T, int K>
+__device__ inline double half_sq_norm(const T (&p)[K]) {
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) s += (double)p[k] * (double)p[k];
+  return 0.5 * wave_sum(s);
+}
+
+template <typename T, int K>
+__device__ inline T dot(const T (&u)[K], const T (&v)[K]) {
+  T s = T(0);
+#pragma unroll
+  for (int k = 0; k < K; ++k) s += u[k] * v[k];
+  return wave_sum(s);
+}
+
+template <typename T, int K>
+__device__ inline void copy(T (&d)[K], const T (&s)[K]) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) d[k] = s[k];
+}
+
+template <typename T, int K>
+__device__ inline void load_row(T (&v)[K], const T* row, int dim) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int e = k * 64 + lane;
+    v[k] = (e < dim) ? row[e] : T(0);
+  }
+}
+
+template <typename T, int K>
+__device__ inline void store_row(T* row, const T (&v)[K], int dim) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int e = k * 64 + lane;
+    if (e < dim) row[e] = v[k];
+  }
+}
+
+// `num_steps` leapfrog steps with the reference op order
+// (leapfrog_integrator.py:280-309, 319-362):
+//   p += (0.5*eps)*g; repeat: q += eps*p; (t, g) = grad(q); p += eps*g;
+//   p -= (0.5*eps)*g.
+template <typename T, int K, int FAM>
+__device__ inline void leapfrog(Target<T, K, FAM>& tg, T eps, int num_steps,
+                                T (&q)[K], T (&p)[K], T (&g)[K], double& t) {
+  const T half = T(0.5) * eps;
+#pragma unroll
+  for (int k = 0; k < K; ++k) p[k] = p[k] + half * g[k];
+  for (int s = 0; s < num_steps; ++s) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) q[k] = q[k] + eps * p[k];
+    t = tg.eval(q, g);
+#pragma unroll
+    for (int k = 0; k < K; ++k) p[k] = p[k] + eps * g[k];
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) p[k] = p[k] - half * g[k];
+}
+
+template <typename T>
+__device__ inline T load_step(const hipmc_run_args& a, int c) {
+  const int idx = a.chain_step ? a.chain_step[c] : 0;
+  return ((const T*)a.step_size)[idx];
+}
+
+template <typename T>
+__device__ inline void put(void* base, size_t i, T v) {
+  if (base) ((T*)base)[i] = v;
+}
+
+// Trace row of launch iteration `it`, or -1 if it is not recorded.
+__device__ inline int64_t trace_row(const hipmc_run_args& a, int it) {
+  if (!a.trace_state) return -1;
+  const int64_t g = a.iter0 + it - a.trace_start;
+  if (g < 0 || g % a.trace_every) return -1;
+  const int64_t r = g / a.trace_every;
+  return r < a.n_trace ? r : -1;
+}
+
+// ------------------------------------------------------------------ NUTS
+// LDS per wave (elements of T): U-turn memory momentum [max_depth][K*64],
+// cumulative momentum [max_depth][K*64], then target scratch [K*64 + 64].
+template <typename T, int K>
+__host__ __device__ constexpr size_t nuts_lds_elems(int max_depth) {
+  return (size_t)2 * max_depth * K * 64 + K * 64 + 64;
+}
+
+template <typename T, int K, int FAM>
+__global__ void __launch_bounds__(256) nuts_kernel(hipmc_run_args a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int lane = lane_id();
+  const int wave = threadIdx.x >> 6;
+  const int waves = blockDim.x >> 6;
+  const int c = blockIdx.x * waves + wave;
+  if (c >= a.n_chains) return;  // whole wave leaves together
+  const int max_depth = a.max_tree_depth;
+  T* wl = (T*)smem_raw + (size_t)wave * nuts_lds_elems<T, K>(max_depth);
+  T* memP = wl;
+  T* memC = wl + (size_t)max_depth * K * 64;
+  T* scratch = wl + (size_t)2 * max_depth * K * 64;
+
+  const ChainId id = chain_of(a, c);
+  const hipmc_problem& pb = a.problems[id.prob];
+  Target<T, K, FAM> tg;
+  tg.load(pb, scratch);
+  const int dim = pb.dim;
+  const size_t row = (size_t)c * a.dim_stride;
+  const double max_de = a.max_energy_diff;
+  const int nunroll = a.unrolled_leapfrog_steps;
+
+  T q0[K], g0[K];
+  load_row(q0, (const T*)a.state + row, dim);
+  load_row(g0, (const T*)a.grad + row, dim);
+  double t0 = (double)((const T*)a.tlp)[c];
+  const T eps = load_step<T>(a, c);
+
+  for (int it = 0; it < a.n_iters; ++it) {
+    const Seed sd = iter_seed(a, it);
+    T pL[K], qL[K], gL[K], pR[K], qR[K], gR[K];
+    draw_momentum(sd, id, dim, pL);
+    const double h0 = t0 - half_sq_norm(pL);  // compute_hamiltonian, -H
+    copy(qL, q0);
+    copy(gL, g0);
+    copy(pR, pL);
+    copy(qR, q0);
+    copy(gR, g0);
+    double tL = t0, tR = t0;
+    // multinomial candidate (nuts.py:361-366)
+    T qC[K], gC[K];
+    copy(qC, q0);
+    copy(gC, g0);
+    double tC = t0, eC = h0, wC = 0.0;
+    T rho[K];
+    copy(rho, pL);  // momentum_sum
+    double eds = 0.0;
+    int lf = 0;
+    bool cont = true, notdiv = true, is_acc = false;
+    int depth = 0;
+    for (; depth < max_depth && cont; ++depth) {
+      // _loop_tree_doubling (nuts.py:541-711)
+      const U4 tw = philox(U4{0u, (uint32_t)id.local, (uint32_t)id.prob,
+                              kPurposeTree | (uint32_t)depth},
+                           sd.k0, sd.k1);
+      const bool dir = (tw.x >> 31) != 0;
+      const double tree_u = (double)uniform_from<T>(tw.y, tw.z);
+      const T e = dir ? eps : -eps;
+      T q[K], p[K], g[K];
+      double t;
+      if (dir) {
+        copy(q, qR); copy(p, pR); copy(g, gR); t = tR;
+      } else {
+        copy(q, qL); copy(p, pL); copy(g, gL); t = tL;
+      }
+      // _build_sub_tree (nuts.py:713-791)
+      T qS[K], gS[K];
+      copy(qS, q);
+      copy(gS, g);
+      double tS = t, eS = t, wS = neg_inf();
+      T cs[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) cs[k] = T(0);
+      double eds_sub = 0.0;
+      int lf_sub = 0;
+      bool csub = true, nd = notdiv;
+      const int nsteps = 1 << depth;
+      for (int i = 0; i < nsteps; ++i) {
+        // _loop_build_sub_tree (nuts.py:793-946)
+        leapfrog(tg, e, nunroll, q, p, g, t);
+        ++lf_sub;
+        if ((i & 1) == 0) {
+          // store (p, cumsum before this step) in slot popcount(i)
+          const int slot = __popc(i);
+          T* mp = memP + (size_t)slot * K * 64;
+          T* mc = memC + (size_t)slot * K * 64;
+#pragma unroll
+          for (int k = 0; k < K; ++k) {
+            mp[k * 64 + lane] = p[k];
+            mc[k * 64 + lane] = cs[k];
+          }
+        }
+#pragma unroll
+        for (int k = 0; k < K; ++k) cs[k] = cs[k] + p[k];
+        bool no_u = true;
+        if (i & 1) {
+          // generalized U-turn against slots [popc(i) - tz(i+1), popc(i))
+          // (has_not_u_turn_at_all_index, nuts.py:949-1010)
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+          const int top = __popc(i);
+          const int lo = top - __builtin_ctz((unsigned)(i + 1));
+          for (int j = lo; j < top && no_u; ++j) {
+            const T* mp = memP + (size_t)j * K * 64;
+            const T* mc = memC + (size_t)j * K * 64;
+            T sl = T(0), sr = T(0);
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+              const T d = cs[k] - mc[k * 64 + lane];
+              sl += d * mp[k * 64 + lane];
+              sr += d * p[k];
+            }
+            const T dl = wave_sum(sl);
+            const T dr = wave_sum(sr);
+            no_u = (dl >= T(0)) && (dr >= T(0));
+          }
+        }
+        double energy = t - half_sq_norm(p);
+        if (__builtin_isnan(energy)) energy = neg_inf();
+        const double de = energy - h0;
+        const bool not_div = -de < max_de;
+        const double wsum = log_add_exp(wS, de);
+        const double thresh = de - wsum;
+        const U4 lw = philox(U4{(uint32_t)i, (uint32_t)id.local,
+                                (uint32_t)id.prob,
+                                kPurposeLeaf | (uint32_t)depth},
+                             sd.k0, sd.k1);
+        const double u = (double)uniform_from<T>(lw.x, lw.y);
+        if (log1p(-u) <= thresh) {
+          copy(qS, q);
+          copy(gS, g);
+          tS = t;
+          eS = energy;
+        }
+        wS = wsum;
+        if (not_div) eds_sub += exp(fmin(de, 0.0));
+        nd = nd && not_div;
+        csub = not_div && no_u;
+        if (!csub) break;
+      }
+      eds += eds_sub;
+      lf += lf_sub;
+      const double tree_w = csub ? wS : neg_inf();
+      const double wsum = log_add_exp(tree_w, wC);
+      double thresh = tree_w - wC;
+      if (__builtin_isnan(thresh)) thresh = 0.0;
+      const bool choose = (log1p(-tree_u) <= thresh) && csub;
+      if (choose) {
+        copy(qC, qS);
+        copy(gC, gS);
+        tC = tS;
+        eC = eS;
+      }
+      wC = wsum;
+      if (dir) {
+        copy(qR, q); copy(pR, p); copy(gR, g); tR = t;
+      } else {
+        copy(qL, q); copy(pL, p); copy(gL, g); tL = t;
+      }
+#pragma unroll
+      for (int k = 0; k < K; ++k) rho[k] = rho[k] + cs[k];
+      const T dl = dot(rho, pL);
+      const T dr = dot(rho, pR);
+      is_acc = is_acc || choose;
+      cont = csub && (dl >= T(0)) && (dr >= T(0));
+      notdiv = nd;
+    }
+    copy(q0, qC);
+    copy(g0, gC);
+    t0 = tC;
+    if (lane == 0) {
+      const size_t o = (size_t)(a.out_row0 + it) * a.n_chains + c;
+      put<T>(a.out_log_accept_ratio, o, (T)log(eds / (double)lf));
+      if (a.out_leapfrogs) a.out_leapfrogs[o] = lf * nunroll;
+      if (a.out_is_accepted) a.out_is_accepted[o] = is_acc;
+      if (a.out_reach_max_depth) a.out_reach_max_depth[o] = cont;
+      if (a.out_has_divergence) a.out_has_divergence[o] = !notdiv;
+      put<T>(a.out_energy, o, (T)eC);
+      if (a.out_tree_depth) a.out_tree_depth[o] = depth;
+    }
+    const int64_t tr = trace_row(a, it);
+    if (tr >= 0)
+      store_row((T*)a.trace_state + ((size_t)tr * a.n_chains + c) * a.dim_stride,
+                q0, dim);
+  }
+  store_row((T*)a.state + row, q0, dim);
+  store_row((T*)a.grad + row, g0, dim);
+  if (lane == 0) ((T*)a.tlp)[c] = (T)t0;
+}
+
+// ------------------------------------------------------------------ HMC
+template <typename T, int K>
+__host__ __device__ constexpr size_t hmc_lds_elems() {
+  return (size_t)K * 64 + 64;
+}
+
+// CALIBRATED=true: full MH(HMC) step; false: proposal only (n_iters == 1).
+template <typename T, int K, int FAM, bool CALIBRATED>
+__global__ void __launch_bounds__(256) hmc_kernel(hipmc_run_args a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int lane = lane_id();
+  const int wave = threadIdx.x >> 6;
+  const int waves = blockDim.x >> 6;
+  const int c = blockIdx.x * waves + wave;
+  if (c >= a.n_chains) return;
+  T* scratch = (T*)smem_raw + (size_t)wave * hmc_lds_elems<T, K>();
+  const ChainId id = chain_of(a, c);
+  const hipmc_problem& pb = a.problems[id.prob];
+  Target<T, K, FAM> tg;
+  tg.load(pb, scratch);
+  const int dim = pb.dim;
+  const size_t row = (size_t)c * a.dim_stride;
+  const int L = a.num_leapfrog_steps;
+
+  T q0[K], g0[K];
+  load_row(q0, (const T*)a.state + row, dim);
+  load_row(g0, (const T*)a.grad + row, dim);
+  double t0 = (double)((const T*)a.tlp)[c];
+  const T eps = load_step<T>(a, c);
+
+  for (int it = 0; it < a.n_iters; ++it) {
+    const Seed sd = iter_seed(a, it);
+    // MH splits the step seed into (proposal, acceptance)
+    // (metropolis_hastings.py:183); the uncalibrated kernel alone uses the
+    // seed it is given as its proposal seed.
+    const Seed prop = CALIBRATED ? split(sd, 0) : sd;
+    T p[K], q[K], g[K];
+    draw_momentum(prop, id, dim, p);
+    T p_init[K];
+    copy(p_init, p);
+    const double ke0 = half_sq_norm(p);
+    copy(q, q0);
+    copy(g, g0);
+    double t = t0;
+    leapfrog(tg, eps, L, q, p, g, t);
+    const double ke1 = half_sq_norm(p);
+    // _compute_log_acceptance_correction: 0.5*safe_sum([sum p0^2, -sum p1^2])
+    double corr = ke0 - ke1;
+    if (!__builtin_isfinite(corr)) corr = neg_inf();
+    const size_t o = (size_t)(a.out_row0 + it) * a.n_chains + c;
+    if (CALIBRATED) {
+      const Seed acc = split(sd, 1);
+      double lar = (t - t0) + corr;  // safe_sum([t1, -t0, corr])
+      if (!__builtin_isfinite(lar)) lar = neg_inf();
+      const U4 uw = philox(U4{0u, (uint32_t)id.local, (uint32_t)id.prob,
+                              kPurposeMHAccept},
+                           acc.k0, acc.k1);
+      const double u = (double)uniform_from<T>(uw.x, uw.y);
+      const bool accept = log(u) < lar;
+      if (it == a.n_iters - 1) {
+        if (a.out_proposed_state) store_row((T*)a.out_proposed_state + row, q, dim);
+        if (a.out_initial_momentum) store_row((T*)a.out_initial_momentum + row, p_init, dim);
+        if (a.out_final_momentum) store_row((T*)a.out_final_momentum + row, p, dim);
+        if (lane == 0) put<T>(a.out_proposed_tlp, c, (T)t);
+      }
+      if (accept) {
+        copy(q0, q);
+        copy(g0, g);
+        t0 = t;
+      }
+      if (lane == 0) {
+        put<T>(a.out_log_accept_ratio, o, (T)lar);
+        if (a.out_is_accepted) a.out_is_accepted[o] = accept;
+        put<T>(a.out_energy, o, (T)corr);
+      }
+    } else {
+      copy(q0, q);
+      copy(g0, g);
+      t0 = t;
+      if (a.out_initial_momentum) store_row((T*)a.out_initial_momentum + row, p_init, dim);
+      if (a.out_final_momentum) store_row((T*)a.out_final_momentum + row, p, dim);
+      if (lane == 0) put<T>(a.out_energy, o, (T)corr);
+    }
+    const int64_t tr = trace_row(a, it);
+    if (tr >= 0)
+      store_row((T*)a.trace_state + ((size_t)tr * a.n_chains + c) * a.dim_stride,
+                q0, dim);
+  }
+  store_row((T*)a.state + row, q0, dim);
+  store_row((T*)a.grad + row, g0, dim);
+  if (lane == 0) ((T*)a.tlp)[c] = (T)t0;
+}
+
+// ------------------------------------------------------------------ eval
+template <typename T, int K, int FAM>
+__global__ void __launch_bounds__(256)
+    eval_kernel(const hipmc_problem* problems, const int32_t* chain_problem,
+                int n_chains, int dim_stride, const T* x, T* out_tlp,
+                T* out_grad) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int wave = threadIdx.x >> 6;
+  const int waves = blockDim.x >> 6;
+  const int c = blockIdx.x * waves + wave;
+  if (c >= n_chains) return;
+  T* scratch = (T*)smem_raw + (size_t)wave * hmc_lds_elems<T, K>();
+  const int prob = chain_problem ? chain_problem[c] : 0;
+  const hipmc_problem& pb = problems[prob];
+  Target<T, K, FAM> tg;
+  tg.load(pb, scratch);
+  const size_t row = (size_t)c * dim_stride;
+  T q[K], g[K];
+  load_row(q, x + row, pb.dim);
+  const double t = tg.eval(q, g);
+  if (out_grad) store_row(out_grad + row, g, pb.dim);
+  if (lane_id() == 0 && out_tlp) out_tlp[c] = (T)t;
+}
+
+}  // namespace hipmc

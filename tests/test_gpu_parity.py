@@ -1,0 +1,386 @@
+"""GPU parity: libhipmc (through the C ABI and the TFP-named Python layer)
+against the CPU oracle on the same seeded inputs.
+
+Tolerances: float32 kernels vs the float64 oracle on float32-rounded inputs
+<= 1e-5 relative on states/log-probs after one transition (2e-4 for whole
+multi-step trajectories), float64 kernels <= 1e-11; accept/reject, tree
+depths, leapfrog counts and divergence flags identical.
+"""
+
+import ctypes
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import mcmc as omcmc
+from oracle import rng as orng
+from oracle import targets as otg
+from probability_amd import _lib
+from probability_amd import bijectors as pbij
+from probability_amd import mcmc as pmcmc
+from probability_amd import targets as ptg
+
+pytestmark = pytest.mark.gpu
+
+DTYPES = [(torch.float32, np.float32, 1e-5), (torch.float64, np.float64, 1e-11)]
+
+
+def _np(t):
+  return t.detach().cpu().numpy()
+
+
+def _case(kind, dtype_np, seed=0, c=64):
+  """(oracle target, product spec ctor, initial state [C, D])."""
+  r = np.random.default_rng(seed)
+  cast = lambda a: np.asarray(a, dtype_np).astype(np.float64)
+  if kind == 'logistic':
+    x = cast(r.normal(size=(200, 7)))
+    y = (r.uniform(size=200) < 0.5).astype(np.float64)
+    o = otg.LogisticRegression(x, y, 1.0)
+    mk = lambda dt: ptg.LogisticRegression(x, y, 1.0, dtype=dt)
+    q = cast(r.normal(size=(c, 7)) * 0.3)
+    return o, mk, q, None
+  if kind == 'linear':
+    x = cast(r.normal(size=(150, 6)))
+    y = cast(x @ r.normal(size=6) + 0.5 * r.normal(size=150))
+    o = otg.LinearRegression(x, y, 1.0, 1.0)
+    mk = lambda dt: ptg.LinearRegression(x, y, 1.0, 1.0, dtype=dt)
+    q = cast(np.concatenate([r.normal(size=(c, 6)) * 0.3,
+                             np.full((c, 1), 0.2)], 1))
+    codes = np.zeros(7, np.int32)
+    codes[6] = 2  # Exp on sigma
+    return o, mk, q, codes
+  if kind == 'hier':
+    j = 70
+    y = cast(4 + 3 * r.normal(size=j) + r.uniform(5, 20, j) * r.normal(size=j))
+    s = cast(r.uniform(5, 20, j))
+    o = otg.HierarchicalNormal(y, s)
+    mk = lambda dt: ptg.HierarchicalNormal(y, s, dtype=dt)
+    q = cast(np.concatenate([r.normal(size=(c, 1)), r.normal(size=(c, 1)),
+                             r.normal(size=(c, j))], 1))
+    codes = np.zeros(j + 2, np.int32)
+    codes[1] = 1  # Softplus on tau
+    return o, mk, q, codes
+  if kind == 'diag':
+    loc = cast(r.normal(size=130))
+    sc = cast(r.uniform(0.5, 2, 130))
+    o = otg.DiagNormal(loc, sc)
+    mk = lambda dt: ptg.DiagNormal(loc, sc, dtype=dt)
+    q = cast(loc + sc * r.normal(size=(c, 130)))
+    return o, mk, q, None
+  if kind == 'funnel':
+    o = otg.NealsFunnel(10)
+    mk = lambda dt: ptg.NealsFunnel(10, dtype=dt)
+    q = cast(r.normal(size=(c, 10)))
+    return o, mk, q, None
+  raise ValueError(kind)
+
+
+def _bijs(spec, codes):
+  if codes is None:
+    return None
+  out = []
+  off = 0
+  for size in spec.part_sizes:
+    c = int(codes[off])
+    out.append([pbij.Identity(), pbij.Softplus(), pbij.Exp()][c])
+    off += size
+  return out
+
+
+def _otarget(o, codes):
+  return o if codes is None else otg.Transformed(o, codes)
+
+
+# ---------------------------------------------------------------- draws
+@pytest.mark.parametrize('tdt,ndt,tol', DTYPES)
+def test_draws_match_oracle(tdt, ndt, tol):
+  lib = _lib.lib()
+  seed = orng.sanitize_seed(1234)
+  sp = np.ascontiguousarray(seed)
+  n, d = 300, 37
+  for problem in (0, 5):
+    out = torch.empty((n, d), dtype=tdt, device='cuda')
+    _lib.check(lib.hipmc_draws(_lib.dtype_code(tdt), 0,
+                               sp.ctypes.data_as(ctypes.c_void_p), problem, n,
+                               d, 0, 0, _lib.ptr(out), None, 0))
+    ref = orng.momentum(seed, np.arange(n), d, ndt, problem)
+    np.testing.assert_allclose(_np(out), ref, rtol=4 * np.finfo(ndt).eps,
+                               atol=4 * np.finfo(ndt).eps)
+    out = torch.empty(n, dtype=tdt, device='cuda')
+    _lib.check(lib.hipmc_draws(_lib.dtype_code(tdt), 1,
+                               sp.ctypes.data_as(ctypes.c_void_p), problem, n,
+                               0, 0, 0, _lib.ptr(out), None, 0))
+    np.testing.assert_array_equal(_np(out),
+                                  orng.mh_uniform(seed, np.arange(n), ndt,
+                                                  problem))
+    aux = torch.empty(n, dtype=torch.uint8, device='cuda')
+    _lib.check(lib.hipmc_draws(_lib.dtype_code(tdt), 2,
+                               sp.ctypes.data_as(ctypes.c_void_p), problem, n,
+                               0, 3, 0, _lib.ptr(out), _lib.ptr(aux), 0))
+    dref, uref = orng.tree_draws(seed, np.arange(n), 3, ndt, problem)
+    np.testing.assert_array_equal(_np(aux).astype(bool), dref)
+    np.testing.assert_array_equal(_np(out), uref)
+    _lib.check(lib.hipmc_draws(_lib.dtype_code(tdt), 3,
+                               sp.ctypes.data_as(ctypes.c_void_p), problem, n,
+                               0, 4, 9, _lib.ptr(out), None, 0))
+    np.testing.assert_array_equal(_np(out), orng.leaf_uniform(
+        seed, np.arange(n), 4, 9, ndt, problem))
+
+
+# ---------------------------------------------------------------- targets
+@pytest.mark.parametrize('tdt,ndt,tol', DTYPES)
+@pytest.mark.parametrize('kind', ['logistic', 'linear', 'hier', 'diag',
+                                  'funnel'])
+def test_target_eval_matches_oracle(kind, tdt, ndt, tol):
+  o, mk, q, codes = _case(kind, ndt)
+  spec = mk(tdt)
+  ot = _otarget(o, codes)
+  x = torch.as_tensor(q, dtype=tdt, device='cuda')
+  tlp, grad = spec.log_prob_and_grad_flat(x, _bijs(spec, codes))
+  rt, rg = ot.log_prob_and_grad(q)
+  np.testing.assert_allclose(_np(tlp), rt, rtol=tol * 5, atol=tol * 5)
+  scale = np.abs(rg).max()
+  np.testing.assert_allclose(_np(grad), rg, rtol=tol * 10, atol=tol * 10 * scale)
+
+
+@pytest.mark.parametrize('tdt,ndt,tol', DTYPES)
+def test_target_callable_like_tfp(tdt, ndt, tol):
+  o, mk, q, _ = _case('hier', ndt, c=8)
+  spec = mk(tdt)
+  parts = [torch.as_tensor(q[:, 0], dtype=tdt), torch.as_tensor(q[:, 1], dtype=tdt),
+           torch.as_tensor(q[:, 2:], dtype=tdt)]
+  tlp = spec(*parts)
+  assert tlp.shape == (8,)
+  np.testing.assert_allclose(_np(tlp), o.log_prob(q), rtol=tol * 5)
+
+
+# ---------------------------------------------------------------- HMC
+@pytest.mark.parametrize('tdt,ndt,tol', DTYPES)
+@pytest.mark.parametrize('kind', ['logistic', 'hier'])
+def test_hmc_one_step_matches_oracle(kind, tdt, ndt, tol):
+  o, mk, q, codes = _case(kind, ndt, seed=3)
+  spec = mk(tdt)
+  ot = _otarget(o, codes)
+  eps, nl = (0.05, 10) if kind == 'logistic' else (0.2, 6)
+  kern = pmcmc.HamiltonianMonteCarlo(
+      spec if codes is None else ptg.TransformedTarget(spec, _bijs(spec, codes)),
+      step_size=eps, num_leapfrog_steps=nl)
+  x = torch.as_tensor(q, dtype=tdt, device='cuda')
+  parts = spec.unpack_state(x, (q.shape[0],))
+  pkr = kern.bootstrap_results(parts)
+  seed = orng.sanitize_seed(77)
+  nxt, res = kern.one_step(parts, pkr, seed=seed)
+  t0, g0 = ot.log_prob_and_grad(q)
+  oq, ores = omcmc.hmc_one_step(ot, np.asarray(eps, ndt), nl, q, t0, g0, seed,
+                                orng.PhiloxDraws(ndt))
+  np.testing.assert_array_equal(_np(res.is_accepted), ores.is_accepted)
+  np.testing.assert_allclose(_np(spec.pack_state(nxt)), oq, rtol=tol * 10,
+                             atol=tol * 10)
+  np.testing.assert_allclose(_np(res.log_accept_ratio), ores.log_accept_ratio,
+                             rtol=tol * 100, atol=tol * 100)
+  np.testing.assert_allclose(_np(spec.pack_state(res.proposed_state)),
+                             ores.proposed_state, rtol=tol * 10, atol=tol * 10)
+
+
+@pytest.mark.parametrize('tdt,ndt,tol', DTYPES)
+def test_mh_over_uncalibrated_equals_fused_hmc(tdt, ndt, tol):
+  o, mk, q, _ = _case('logistic', ndt, seed=4, c=32)
+  spec = mk(tdt)
+  x = torch.as_tensor(q, dtype=tdt, device='cuda')
+  parts = spec.unpack_state(x, (32,))
+  fused = pmcmc.HamiltonianMonteCarlo(spec, 0.05, 8)
+  comp = pmcmc.MetropolisHastings(
+      pmcmc.UncalibratedHamiltonianMonteCarlo(spec, 0.05, 8))
+  pa, pb = fused.bootstrap_results(parts), comp.bootstrap_results(parts)
+  sa, sb = parts, parts
+  for s in range(3):
+    sa, pa = fused.one_step(sa, pa, seed=orng.fold_in(5, s))
+    sb, pb = comp.one_step(sb, pb, seed=orng.fold_in(5, s))
+    np.testing.assert_array_equal(_np(pa.is_accepted), _np(pb.is_accepted))
+    np.testing.assert_array_equal(_np(sa[0]), _np(sb[0]))
+
+
+def test_hmc_nan_rejection_on_gpu():
+  spec = ptg.NaNExponential(10, dtype=torch.float32)
+  x = torch.linspace(0.01, 5, 10, device='cuda')[None]
+  kern = pmcmc.HamiltonianMonteCarlo(spec, step_size=2., num_leapfrog_steps=5)
+  nxt, res = kern.one_step(x, kern.bootstrap_results(x), seed=17)
+  np.testing.assert_array_equal(_np(nxt), _np(x))
+  assert float(torch.exp(torch.clamp(res.log_accept_ratio, max=0.0))[0]) == 0.0
+
+
+# ---------------------------------------------------------------- NUTS
+def _nuts_one(kind, tdt, ndt, seed=0, c=64, eps=None, max_depth=10):
+  o, mk, q, codes = _case(kind, ndt, seed=seed, c=c)
+  spec = mk(tdt)
+  ot = _otarget(o, codes)
+  eps = eps or {'logistic': 0.08, 'linear': 0.05, 'hier': 0.3, 'diag': 0.5,
+                'funnel': 0.9}[kind]
+  tspec = spec if codes is None else ptg.TransformedTarget(spec,
+                                                           _bijs(spec, codes))
+  kern = pmcmc.NoUTurnSampler(tspec, step_size=np.asarray(eps, ndt),
+                              max_tree_depth=max_depth)
+  x = torch.as_tensor(q, dtype=tdt, device='cuda')
+  parts = spec.unpack_state(x, (c,))
+  pkr = kern.bootstrap_results(parts)
+  seed_arr = orng.sanitize_seed(1000 + seed)
+  nxt, res = kern.one_step(parts, pkr, seed=seed_arr)
+  t0, g0 = ot.log_prob_and_grad(q)
+  oq, ores = omcmc.nuts_one_step(ot, float(np.asarray(eps, ndt)), q, t0, g0,
+                                 seed_arr, orng.PhiloxDraws(ndt),
+                                 max_depth=max_depth)
+  return spec, nxt, res, oq, ores
+
+
+@pytest.mark.parametrize('tdt,ndt,tol', DTYPES)
+@pytest.mark.parametrize('kind', ['hier', 'logistic', 'linear', 'diag',
+                                  'funnel'])
+def test_nuts_one_step_matches_oracle(kind, tdt, ndt, tol):
+  for seed in range(3):
+    spec, nxt, res, oq, ores = _nuts_one(kind, tdt, ndt, seed=seed)
+    np.testing.assert_array_equal(_np(res.leapfrogs_taken),
+                                  ores.leapfrogs_taken)
+    np.testing.assert_array_equal(_np(res.is_accepted), ores.is_accepted)
+    np.testing.assert_array_equal(_np(res.reach_max_depth),
+                                  ores.reach_max_depth)
+    np.testing.assert_array_equal(_np(res.has_divergence),
+                                  ores.has_divergence)
+    ftol = tol * 20
+    np.testing.assert_allclose(_np(spec.pack_state(nxt)), oq, rtol=ftol,
+                               atol=ftol)
+    fin = np.isfinite(ores.log_accept_ratio)
+    np.testing.assert_allclose(_np(res.log_accept_ratio)[fin],
+                               ores.log_accept_ratio[fin], rtol=ftol * 10,
+                               atol=ftol * 10)
+    np.testing.assert_allclose(_np(res.target_log_prob), ores.target_log_prob,
+                               rtol=ftol, atol=ftol)
+
+
+def test_nuts_divergence_on_gpu_funnel():
+  # `mcmc/nuts_test.py:445-474`
+  spec = ptg.NealsFunnel(10, dtype=torch.float32)
+  r = np.random.default_rng(0)
+  v = 3 * r.normal(size=(5, 1))
+  x = np.concatenate([v, np.exp(v / 2) * r.normal(size=(5, 9))], 1)
+  parts = spec.unpack_state(torch.as_tensor(x, dtype=torch.float32,
+                                            device='cuda'), (5,))
+  kern = pmcmc.NoUTurnSampler(lambda_free(spec), step_size=1.0)
+  _, div = pmcmc.sample_chain(100, parts, kernel=kern, seed=3,
+                              trace_fn=pmcmc.TraceFields('has_divergence'))
+  assert int(div.sum()) > 100
+
+
+def lambda_free(spec):
+  return spec
+
+
+# ---------------------------------------------------------------- DA
+@pytest.mark.parametrize('tdt,ndt,tol', DTYPES)
+def test_da_kernel_matches_oracle(tdt, ndt, tol):
+  lar = np.log([[0.70, 0.76, 0.73], [0.76, 0.76, 0.73]]).astype(ndt)
+  for step_shape in [(), (3, 1), (2, 3, 1), (2, 1, 1), (1, 1)]:
+    step = np.ones(step_shape)
+
+    class FakeInner:
+      pass
+    # drive the product DA with a fake inner kernel
+    from probability_amd.mcmc import kernel as pk
+    import collections
+    R = collections.namedtuple('R', 'log_accept_ratio, step_size')
+
+    class Fake(pk.TransitionKernel):
+      def __init__(self, step_size, store_parameters_in_results=False):
+        self._parameters = dict(step_size=step_size,
+                                store_parameters_in_results=True)
+
+      def one_step(self, s, pkr, seed=None):
+        return s, pkr
+
+      def bootstrap_results(self, s):
+        return R(torch.as_tensor(lar, device='cuda'),
+                 torch.as_tensor(self._parameters['step_size'], dtype=tdt,
+                                 device='cuda'))
+
+      @property
+      def is_calibrated(self):
+        return True
+
+    k = pmcmc.DualAveragingStepSizeAdaptation(Fake(step), num_adaptation_steps=1)
+    state = torch.zeros((2, 3, 4), dtype=tdt, device='cuda')
+    pkr = k.bootstrap_results(state)
+    for _ in range(2):
+      _, pkr = k.one_step(state, pkr)
+    ost = omcmc.da_bootstrap(step)
+    for _ in range(2):
+      ost = omcmc.da_update(ost, lar.astype(np.float64), state_rank=3,
+                            num_adaptation_steps=1)
+    np.testing.assert_allclose(_np(pkr.inner_results.step_size),
+                               np.broadcast_to(ost.step_size, step_shape),
+                               rtol=max(tol, 1e-6))
+
+
+# ---------------------------------------------------------------- drivers
+def _hier_kernel(tdt, ndt, j=40, c=64, adapt=10):
+  r = np.random.default_rng(11)
+  y = (4 + 3 * r.normal(size=j) + r.uniform(5, 20, j) * r.normal(size=j))
+  s = r.uniform(5, 20, j)
+  spec = ptg.HierarchicalNormal(np.asarray(y, ndt), np.asarray(s, ndt),
+                                dtype=tdt)
+  nuts = pmcmc.NoUTurnSampler(spec, step_size=np.asarray(0.1, ndt))
+  ttk = pmcmc.TransformedTransitionKernel(
+      nuts, [pbij.Identity(), pbij.Softplus(), pbij.Identity()])
+  da = pmcmc.DualAveragingStepSizeAdaptation(ttk, num_adaptation_steps=adapt,
+                                             target_accept_prob=0.8)
+  init = [torch.zeros(c, dtype=tdt), torch.full((c,), float(np.log(2.0)),
+                                                dtype=tdt),
+          torch.zeros((c, j), dtype=tdt)]
+  return spec, da, init, y, s
+
+
+@pytest.mark.parametrize('tdt,ndt,tol', DTYPES)
+def test_fused_sample_chain_equals_step_loop(tdt, ndt, tol):
+  spec, da, init, _, _ = _hier_kernel(tdt, ndt)
+  tf = pmcmc.TraceFields('log_accept_ratio', 'leapfrogs_taken', 'step_size')
+  a = pmcmc.sample_chain(12, init, kernel=da, num_burnin_steps=8,
+                         num_steps_between_results=1, trace_fn=tf, seed=5,
+                         return_final_kernel_results=True)
+  b = pmcmc.sample_chain(12, init, kernel=da, num_burnin_steps=8,
+                         num_steps_between_results=1, trace_fn=tf, seed=5,
+                         return_final_kernel_results=True,
+                         experimental_fused=False)
+  for pa, pb in zip(a.all_states, b.all_states):
+    np.testing.assert_array_equal(_np(pa), _np(pb))
+  np.testing.assert_array_equal(_np(a.trace[0]), _np(b.trace[0]))
+  np.testing.assert_array_equal(_np(a.trace[1]), _np(b.trace[1]))
+  np.testing.assert_allclose(_np(a.trace[2]).reshape(-1),
+                             _np(b.trace[2]).reshape(-1), rtol=0)
+  np.testing.assert_array_equal(_np(a.final_kernel_results.new_step_size),
+                                _np(b.final_kernel_results.new_step_size))
+
+
+@pytest.mark.parametrize('tdt,ndt,tol', DTYPES)
+def test_sample_chain_nuts_da_ttk_matches_oracle(tdt, ndt, tol):
+  spec, da, init, y, s = _hier_kernel(tdt, ndt, j=12, c=32, adapt=6)
+  tf = pmcmc.TraceFields('log_accept_ratio', 'leapfrogs_taken')
+  res = pmcmc.sample_chain(4, init, kernel=da, num_burnin_steps=6, trace_fn=tf,
+                           seed=21)
+  base = otg.HierarchicalNormal(np.asarray(y, ndt).astype(np.float64),
+                                np.asarray(s, ndt).astype(np.float64))
+  codes = np.zeros(14, np.int32)
+  codes[1] = 1
+  ot = otg.Transformed(base, codes)
+  x0 = np.zeros((32, 14))
+  ox, otrace, _ = omcmc.sample_nuts_da(ot, x0, np.asarray(0.1, ndt), 6, 4, 6,
+                                       seed=orng.sanitize_seed(21),
+                                       target_accept_prob=0.8, dtype=ndt)
+  # the oracle returns unconstrained states; compare constrained
+  ostates = np.stack([ot.forward(v) for v in ox])
+  got = np.concatenate([_np(res.all_states[0])[..., None],
+                        _np(res.all_states[1])[..., None],
+                        _np(res.all_states[2])], -1)
+  lf = np.stack([r.leapfrogs_taken for r in otrace])
+  np.testing.assert_array_equal(_np(res.trace[1]), lf)
+  np.testing.assert_allclose(got, ostates, rtol=2e-4 if ndt == np.float32
+                             else 1e-9, atol=2e-4 if ndt == np.float32 else 1e-9)
