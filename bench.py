@@ -1,0 +1,242 @@
+"""Benchmark: min-ESS/s at split-R-hat < 1.01 for NUTS + dual averaging +
+Softplus TTK on the non-centred hierarchical normal (J = 256, 1024 chains,
+1000 adaptation + 2000 draws, float32) on one MI355X.
+
+One "step" is one full sampling run (bootstrap, 1000 adapted NUTS
+transitions, 2000 recorded transitions) with the data already in HBM.
+`value` = min over the 258 parameters of the cross-chain ESS of the 2000
+draws, divided by the wall time of one run.
+
+Also reported: the plain execution of the same chain (host step loop, one
+NUTS launch + one DA launch per transition) timed once, the NUTS kernel's
+achieved FP32 rate against the MI355X FP32 peak (HIP events over every
+sampling launch), and the CPU restatement of TFP's batched algorithm timed
+on a bounded sample of transitions on one host core.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FP32_PEAK_TFLOPS = 157.3
+# Algorithmic FP32 work of one chain-leapfrog on the hierarchical normal:
+# 13 FLOP per group for log-prob + gradient (theta, residual, two scalings,
+# the two quadratic terms, grad_z, the two reductions), 8 FLOP per state
+# element for the leapfrog update and 2 per element for the kinetic energy.
+def flops_per_leapfrog(j):
+  return 13 * j + 10 * (j + 2)
+
+
+def parse():
+  ap = argparse.ArgumentParser()
+  ap.add_argument('--gpus', type=int, default=1)
+  ap.add_argument('--steps', type=int, default=3)
+  ap.add_argument('--warmup', type=int, default=1)
+  ap.add_argument('--cpu-seconds', type=float, default=12.0)
+  ap.add_argument('--no-cpu', action='store_true')
+  ap.add_argument('--no-plain', action='store_true')
+  ap.add_argument('--seed', type=int, default=20241)
+  return ap.parse_args()
+
+
+def build_kernel(wl, torch, pa):
+  spec = pa.targets.HierarchicalNormal(wl['y'], wl['sigma'],
+                                       dtype=torch.float32, device='cuda')
+  nuts = pa.mcmc.NoUTurnSampler(spec, step_size=np.float32(wl['step_size']),
+                                max_tree_depth=wl['max_tree_depth'])
+  ttk = pa.mcmc.TransformedTransitionKernel(
+      nuts, [pa.bijectors.Identity(), pa.bijectors.Softplus(),
+             pa.bijectors.Identity()])
+  da = pa.mcmc.DualAveragingStepSizeAdaptation(
+      ttk, num_adaptation_steps=wl['num_adaptation_steps'],
+      target_accept_prob=wl['target_accept_prob'])
+  c, j = wl['chains'], len(wl['y'])
+  init = [torch.zeros(c, device='cuda'),
+          torch.full((c,), float(np.log(2.0)), device='cuda'),
+          torch.zeros((c, j), device='cuda')]
+  return spec, da, init
+
+
+def run_once(pa, da, init, wl, seed, fused=True, stats=None):
+  return pa.mcmc.sample_chain(
+      wl['num_results'], init, kernel=da,
+      num_burnin_steps=wl['num_burnin_steps'],
+      trace_fn=pa.mcmc.TraceFields('step_size'), seed=seed,
+      return_final_kernel_results=True, experimental_fused=fused,
+      experimental_stats=stats)
+
+
+def ess_rhat(torch, pa, states):
+  mu, tau, z = states
+  flat = torch.cat([mu[..., None], tau[..., None], z], -1)  # [R, C, D]
+  ess = []
+  rhat = []
+  for s in range(0, flat.shape[-1], 32):
+    blk = flat[..., s:s + 32]
+    ess.append(pa.mcmc.effective_sample_size(blk, cross_chain_dims=1))
+    rhat.append(pa.mcmc.potential_scale_reduction(blk, split_chains=True))
+  ess = torch.cat(ess)
+  rhat = torch.cat(rhat)
+  return flat, ess, rhat
+
+
+def cpu_baseline(wl, final_step, final_x, min_ess, budget_s):
+  """TFP's batched NUTS algorithm restated in NumPy (the oracle), timed on
+  one host core on a bounded number of post-adaptation transitions."""
+  from oracle import mcmc as omcmc
+  from oracle import rng as orng
+  from oracle import targets as otg
+  j = len(wl['y'])
+  codes = np.zeros(j + 2, np.int32)
+  codes[1] = 1
+  ot = otg.Transformed(otg.HierarchicalNormal(wl['y'].astype(np.float64),
+                                              wl['sigma'].astype(np.float64)),
+                       codes)
+  x = final_x.astype(np.float64)
+  t, g = ot.log_prob_and_grad(x)
+  draws = orng.PhiloxDraws(np.float32)
+  n_iter = 0
+  lf = 0
+  t0 = time.perf_counter()
+  while time.perf_counter() - t0 < budget_s:
+    x, r = omcmc.nuts_one_step(ot, final_step, x, t, g,
+                               orng.fold_in(7, n_iter), draws,
+                               max_depth=wl['max_tree_depth'])
+    t, g = r.target_log_prob, r.grads_target_log_prob
+    lf += int(r.leapfrogs_taken.sum())
+    n_iter += 1
+  el = time.perf_counter() - t0
+  per_iter = el / n_iter
+  total_iters = wl['num_adaptation_steps'] + wl['num_results']
+  return dict(value=float(min_ess / (per_iter * total_iters)),
+              unit='min-ESS/s', cores=1, kind='port',
+              sample=(f'{n_iter} post-adaptation NUTS transitions of the '
+                      f'{wl["chains"]}-chain batch ({el:.1f} s, '
+                      f'{lf / el:.0f} chain-grads/s), extrapolated to the '
+                      f'{total_iters}-transition run with the GPU run\'s '
+                      'min-ESS; NumPy float64, 1 thread'),
+              chain_grads_per_s=float(lf / el))
+
+
+def main():
+  args = parse()
+  if args.gpus != 1:
+    print(f'bench.py: this build runs on one GPU; --gpus {args.gpus} is not '
+          'supported (chains and posteriors shard across GPUs with no '
+          'communication; see DESIGN.md)', file=sys.stderr)
+    sys.exit(2)
+  os.environ.setdefault('OMP_NUM_THREADS', '1')
+  import torch
+  import probability_amd as pa
+  from probability_amd import workloads
+
+  wl = workloads.hier_normal_j256()
+  j, c = len(wl['y']), wl['chains']
+  spec, da, init = build_kernel(wl, torch, pa)
+
+  for w in range(args.warmup):
+    run_once(pa, da, init, wl, args.seed + 1000 + w)
+  torch.cuda.synchronize()
+  times = []
+  res = None
+  for k in range(args.steps):
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = run_once(pa, da, init, wl, args.seed + k)
+    torch.cuda.synchronize()
+    times.append(time.perf_counter() - t0)
+  ms = 1000 * float(np.mean(times))
+  flat, ess, rhat = ess_rhat(torch, pa, res.all_states)
+  min_ess = float(ess.min())
+  rhat_max = float(rhat.max())
+
+  # posterior check against float64 quadrature (mu, tau)
+  mom = workloads.hier_posterior_moments(wl['y'], wl['sigma'])
+  draws_mu = flat[..., 0].double()
+  draws_tau = flat[..., 1].double()
+  z_mu = (float(draws_mu.mean()) - mom['mean_mu']) / np.sqrt(
+      mom['var_mu'] / float(ess[0]))
+  z_tau = (float(draws_tau.mean()) - mom['mean_tau']) / np.sqrt(
+      mom['var_tau'] / float(ess[1]))
+
+  # kernel timing (untimed extra run with HIP events around each launch)
+  stats = {'time_kernels': True}
+  run_once(pa, da, init, wl, args.seed, stats=stats)
+  launches = stats['launches']
+  lf_per_step = stats['leapfrogs_per_step'].double().cpu().numpy()
+  kern_ms = np.array([l[0] for l in launches])
+  flops = []
+  for (ms_k, t0_, n_) in launches:
+    flops.append(lf_per_step[t0_:t0_ + n_].sum() * flops_per_leapfrog(j))
+  flops = np.array(flops)
+  achieved = float(flops.sum() / (kern_ms.sum() * 1e-3) / 1e12)
+  adapt_ms = kern_ms[:-1] if len(kern_ms) > 1 else kern_ms
+  grad_evals = float(stats['leapfrogs_total'])
+
+  line = {
+      'metric': 'min-ESS/sec at split-Rhat<1.01 (NUTS+DA+TTK, hierarchical '
+                'normal J=256, 1024 chains)',
+      'value': min_ess / (ms / 1000.0),
+      'unit': 'min-ESS/s',
+      'n_gpus': 1,
+      'steps': args.steps,
+      'warmup': args.warmup,
+      'ms_per_step': ms,
+      'higher_is_better': True,
+      'scaling': 'weak',
+      'vs_baseline': None,
+      'dtype': 'f32',
+      'data': 'synthetic (seeded NumPy PCG64)',
+      'config': {
+          'workload': 'hier_normal_j256: NUTS(max_tree_depth=10) + '
+                      'DualAveraging(target 0.8, 1000 steps) + TTK(Softplus '
+                      'on tau), non-centred hierarchical normal J=256, 1024 '
+                      'chains, 1000 adaptation + 2000 draws',
+          'chains': c, 'groups': j, 'num_adaptation_steps': 1000,
+          'num_results': 2000, 'parallelism': 'single GPU, chains batched'},
+      'min_ess': min_ess,
+      'rhat_max': rhat_max,
+      'rhat_ok': rhat_max < 1.01,
+      'posterior_z_vs_quadrature': {'mu': z_mu, 'tau': z_tau},
+      'chain_grad_evals_per_s': grad_evals / (ms / 1000.0),
+      'final_step_size': float(res.final_kernel_results.new_step_size),
+      'adapt_iteration_ms_p50_p99': [float(np.percentile(adapt_ms, 50)),
+                                     float(np.percentile(adapt_ms, 99))],
+      'roofline': {
+          'bound': 'mfma', 'achieved': achieved, 'peak': FP32_PEAK_TFLOPS,
+          'unit': 'TFLOP/s', 'frac': achieved / FP32_PEAK_TFLOPS,
+          'traffic': None,
+          'kernel': 'nuts_kernel<float,5,hier>',
+          'avg_launch_ms': float(kern_ms.mean()),
+          'launches': len(kern_ms),
+          'flop_per_chain_leapfrog': flops_per_leapfrog(j)},
+  }
+  if not args.no_plain:
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    run_once(pa, da, init, wl, args.seed + args.steps - 1, fused=False)
+    torch.cuda.synchronize()
+    tp = time.perf_counter() - t0
+    line['plain'] = {'value': min_ess / tp, 'ms_per_step': 1000 * tp,
+                     'what': 'same chain via the host step loop (one NUTS + '
+                             'one DA launch per transition)'}
+    line['batched_over_plain'] = (min_ess / (ms / 1000.0)) / (min_ess / tp)
+  if not args.no_cpu:
+    final = res.final_kernel_results
+    xs = spec.pack_state(final.inner_results.transformed_state).cpu().numpy()
+    line['cpu_baseline'] = cpu_baseline(
+        wl, float(final.new_step_size), xs, min_ess, args.cpu_seconds)
+  print(json.dumps(line), flush=True)
+
+
+if __name__ == '__main__':
+  main()

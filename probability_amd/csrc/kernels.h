@@ -330,7 +330,9 @@ __global__ void __launch_bounds__(256) nuts_kernel(hipmc_run_args a) {
     }
     copy(q0, qC);
     copy(g0, gC);
-    t0 = tC;
+    // the log-prob crosses iterations in the state dtype, exactly as it does
+    // between separate one_step launches
+    t0 = (double)(T)tC;
     if (lane == 0) {
       const size_t o = (size_t)(a.out_row0 + it) * a.n_chains + c;
       put<T>(a.out_log_accept_ratio, o, (T)log(eds / (double)lf));
@@ -419,7 +421,7 @@ __global__ void __launch_bounds__(256) hmc_kernel(hipmc_run_args a) {
       if (accept) {
         copy(q0, q);
         copy(g0, g);
-        t0 = t;
+        t0 = (double)(T)t;
       }
       if (lane == 0) {
         put<T>(a.out_log_accept_ratio, o, (T)lar);
@@ -429,7 +431,7 @@ __global__ void __launch_bounds__(256) hmc_kernel(hipmc_run_args a) {
     } else {
       copy(q0, q);
       copy(g0, g);
-      t0 = t;
+      t0 = (double)(T)t;
       if (a.out_initial_momentum) store_row((T*)a.out_initial_momentum + row, p_init, dim);
       if (a.out_final_momentum) store_row((T*)a.out_final_momentum + row, p, dim);
       if (lane == 0) put<T>(a.out_energy, o, (T)corr);

@@ -99,7 +99,8 @@ def sample_chain(num_results, current_state, previous_kernel_results=None,
                  kernel=None, num_burnin_steps=0, num_steps_between_results=0,
                  trace_fn=_trace_kernel_results,
                  return_final_kernel_results=False, parallel_iterations=10,
-                 seed=None, name=None, experimental_fused=True):
+                 seed=None, name=None, experimental_fused=True,
+                 experimental_stats=None):
   """Runs `num_burnin_steps + num_results*(1+num_steps_between_results)`
   steps of `kernel` and returns the post-burn-in states."""
   del parallel_iterations, name
@@ -119,7 +120,8 @@ def sample_chain(num_results, current_state, previous_kernel_results=None,
     if plan is not None:
       out = _run_fused(plan, num_results, current_state, num_burnin_steps,
                        num_steps_between_results, trace_fn,
-                       return_final_kernel_results, base_seed)
+                       return_final_kernel_results, base_seed,
+                       stats=experimental_stats)
       if out is not None:
         return out
   return _run_steps(kernel, num_results, current_state,
@@ -199,7 +201,11 @@ def _inner_results_of(plan, pkr):
 
 
 def _run_fused(plan, num_results, current_state, num_burnin_steps, thin,
-               trace_fn, return_final, base_seed, max_iters_per_launch=4096):
+               trace_fn, return_final, base_seed, max_iters_per_launch=4096,
+               stats=None):
+  """`stats` (a dict, optional) receives leapfrog totals and, if
+  stats['time_kernels'] is set, per-launch HIP-event durations of the
+  sampling kernel."""
   kernel = plan.outer
   pkr = kernel.bootstrap_results(current_state)
   base_res = _inner_results_of(plan, pkr)
@@ -240,8 +246,10 @@ def _run_fused(plan, num_results, current_state, num_burnin_steps, thin,
                          ('reach_max_depth', 'out_reach_max_depth', torch.uint8),
                          ('has_divergence', 'out_has_divergence', torch.uint8),
                          ('energy', 'out_energy', dt)):
-      if name in want or return_final:
+      if name in want or return_final or (stats is not None and
+                                          f == 'out_leapfrogs'):
         outs[f] = torch.empty((rows, c), dtype=tdt, device=dev)
+  events = [] if (stats is not None and stats.get('time_kernels')) else None
   trace_state = torch.empty((num_results, c, d), dtype=dt, device=dev)
   step_hist = (torch.empty((rows,) + tuple(step.shape), dtype=dt, device=dev)
                if 'step_size' in want else None)
@@ -290,7 +298,14 @@ def _run_fused(plan, num_results, current_state, num_burnin_steps, thin,
     a.trace_start = num_burnin_steps
     a.trace_every = 1 + thin
     a.n_trace = num_results
+    if events is not None:
+      e0 = torch.cuda.Event(enable_timing=True)
+      e1 = torch.cuda.Event(enable_timing=True)
+      e0.record()
     _lib.check(fn(a, stream))
+    if events is not None:
+      e1.record()
+      events.append((e0, e1, t, n))
     if tlp_hist is not None and n == 1:
       tlp_hist[t] = tlp
     if plan.da is not None:
@@ -303,6 +318,16 @@ def _run_fused(plan, num_results, current_state, num_burnin_steps, thin,
           _lib.ptr(step), _lib.ptr(step64), stream))
     t += n
 
+  if stats is not None:
+    if 'out_leapfrogs' in outs:
+      stats['leapfrogs_total'] = int(outs['out_leapfrogs'][:n_total].sum())
+      stats['leapfrogs_per_step'] = outs['out_leapfrogs'][:n_total].sum(1)
+    stats['n_steps'] = n_total
+    stats['n_chains'] = c
+    if events is not None:
+      torch.cuda.synchronize(dev)
+      stats['launches'] = [(e0.elapsed_time(e1), t0, n)
+                           for e0, e1, t0, n in events]
   # ---- assemble outputs in the reference's structures
   flat_states = trace_state.reshape(num_results * c, d)
   if plan.ttk is not None:
