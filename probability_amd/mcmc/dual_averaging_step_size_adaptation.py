@@ -184,7 +184,8 @@ class DualAveragingStepSizeAdaptation(kernel_base.TransitionKernel):
 
   def _update_part(self, step_size, state, err, avg, lst, lap, target, step_no,
                    pkr):
-    step_size = torch.as_tensor(step_size)
+    if not isinstance(step_size, torch.Tensor):
+      step_size = torch.as_tensor(np.asarray(step_size), device=lap.device)
     device = lap.device
     dtype = step_size.dtype if step_size.is_floating_point() else lap.dtype
     if dtype not in (torch.float32, torch.float64):
@@ -233,13 +234,15 @@ class DualAveragingStepSizeAdaptation(kernel_base.TransitionKernel):
     device = lap.device
     errs, avgs, lsts = [], [], []
     for ss, sh in zip(step_parts, shrink_parts):
-      ss = torch.as_tensor(ss).to(device=device, dtype=torch.float64)
+      ss = (ss.to(device=device, dtype=torch.float64)
+            if isinstance(ss, torch.Tensor) else
+            torch.as_tensor(np.asarray(ss, np.float64), device=device))
       errs.append(torch.zeros_like(ss))
       avgs.append(torch.zeros_like(ss))
       if sh is None:
         lsts.append(math.log(10.0) + torch.log(ss))
       else:
-        lsts.append(torch.log(torch.as_tensor(sh, dtype=torch.float64,
+        lsts.append(torch.log(torch.as_tensor(np.asarray(sh, np.float64),
                                               device=device)) +
                     torch.zeros_like(ss))
     pack = (lambda v: v) if is_list else (lambda v: v[0])
@@ -266,6 +269,10 @@ def _flat(x):
 
 
 def _f64(v, shape, device):
-  t = torch.as_tensor(v)
-  t = t.to(device=device, dtype=torch.float64)
+  # Python / NumPy scalars go through float64 NumPy: torch.as_tensor would
+  # round them to float32 first.
+  if isinstance(v, torch.Tensor):
+    t = v.to(device=device, dtype=torch.float64)
+  else:
+    t = torch.as_tensor(np.asarray(v, np.float64), device=device)
   return t.expand(shape).reshape(-1) if t.dim() <= len(shape) else t.reshape(-1)

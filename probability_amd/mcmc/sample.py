@@ -263,8 +263,7 @@ def _run_fused(plan, num_results, current_state, num_burnin_steps, thin,
     avg = da_lib._f64(pkr.log_averaging_step, step_shape, dev).clone()
     lst = da_lib._f64(pkr.log_shrinkage_target, step_shape, dev).contiguous()
     tgt = da_lib._f64(pkr.target_accept_prob, step_shape, dev).contiguous()
-    step64 = torch.as_tensor(step0).to(device=dev, dtype=torch.float64)
-    step64 = step64.reshape(-1).clone()
+    step64 = da_lib._f64(step0, step_shape, dev).clone()
     state_rank = len(spec.event_shapes[0]) + len(cs)
     gplan = plan.da._plan(cs, step_shape, state_rank, dev)
     if not gplan.identity:
