@@ -99,8 +99,29 @@ typedef struct hipmc_problem {
   int32_t bij_off[HIPMC_MAX_BIJ_RANGES];
   int32_t bij_len[HIPMC_MAX_BIJ_RANGES];
   int32_t bij_code[HIPMC_MAX_BIJ_RANGES];
-  int32_t pad_;
+  uint32_t key; /* problem word of the Philox counter (stable id) */
 } hipmc_problem;
+
+/*
+ * Optional per-problem sampling stream (device array indexed like the
+ * problem table).  When hipmc_run_args.streams is set, problem p's chains
+ * run min(n_iters, streams[p].n_iters) iterations of the launch, iteration
+ * i uses fold_in(streams[p].seed, streams[p].iter0 + i), and if
+ * streams[p].trace is set the post-step state of recorded iterations
+ * (g - trace_start a non-negative multiple of trace_every, row < n_trace)
+ * goes to trace[row][local_chain][0:dim] (dense per problem).  This lets
+ * posteriors at different points of their runs share one launch.
+ */
+typedef struct hipmc_stream {
+  uint32_t seed[2];
+  int64_t iter0;
+  int32_t n_iters;
+  int32_t trace_every;
+  void* trace;
+  int64_t trace_start;
+  int32_t n_trace;
+  int32_t pad_;
+} hipmc_stream;
 
 /*
  * Arguments of the sampling kernels.  Rows of the state arrays are chains
@@ -168,6 +189,9 @@ typedef struct hipmc_run_args {
   void* out_proposed_tlp;     /* [n_chains] */
   void* out_initial_momentum;
   void* out_final_momentum;
+  /* optional per-problem streams (device, [n_problems]); NULL: launch-level
+   * seed / iter0 / trace for every problem */
+  const hipmc_stream* streams;
 } hipmc_run_args;
 
 const char* hipmc_last_error(void);
@@ -187,6 +211,7 @@ int hipmc_philox_host(const uint32_t ctr[4], const uint32_t key[2],
                       uint32_t out[4]);
 int hipmc_sizeof_problem(void);
 int hipmc_sizeof_run_args(void);
+int hipmc_sizeof_stream(void);
 
 /* Fused calibrated HMC (leapfrog + MH) for n_iters iterations. */
 int hipmc_hmc_run(const hipmc_run_args* args, uint64_t stream);
@@ -231,24 +256,29 @@ int hipmc_da_update(int32_t dtype, int32_t n_steps, const int32_t* step_group,
 /*
  * Metropolis-Hastings accept for n_chains chains:
  *   lar = safe_sum(proposed_tlp - current_tlp [+ correction]);
- *   accept = log(U) < lar, U from the MH purpose keyed by `seed`.
+ *   accept = log(U) < lar, U from the MH purpose keyed by `seed` (counter
+ *   problem word = problems[chain_problem[c]].key).
  * Rows of `current` (state [n, dim_stride], tlp, grad) are overwritten by the
  * proposal where accepted.  correction may be NULL.
  */
 int hipmc_mh_accept(int32_t dtype, int32_t n_chains, int32_t dim_stride,
-                    const uint32_t seed[2], const int32_t* chain_problem,
+                    const uint32_t seed[2], const hipmc_problem* problems,
+                    const int32_t* chain_problem,
                     const int32_t* chain_local, const void* proposed_state,
                     const void* proposed_tlp, const void* proposed_grad,
                     const void* correction, void* state, void* tlp,
                     void* grad, void* out_log_accept_ratio,
                     uint8_t* out_is_accepted, uint64_t stream);
 
-/* Elementwise bijector on x [n_chains, dim_stride] per the problem's bijector
- * ranges: direction 0 = forward, 1 = inverse. */
+/* Elementwise bijector on x [n_rows, dim_stride] per the problem's bijector
+ * ranges: direction 0 = forward, 1 = inverse.  Row r is chain
+ * (chain_period > 0 ? r % chain_period : r) of chain_problem (a trace of
+ * [draws, chains] rows uses chain_period = chains). */
 int hipmc_bijector(int32_t dtype, const hipmc_problem* problems,
-                   const int32_t* chain_problem, int32_t n_chains,
-                   int32_t dim_stride, int32_t direction, const void* x,
-                   void* out, uint64_t stream);
+                   const int32_t* chain_problem, int32_t n_rows,
+                   int32_t dim_stride, int32_t chain_period,
+                   int32_t direction, const void* x, void* out,
+                   uint64_t stream);
 
 /*
  * Raw draws of the schedule, for tests and for callers that need the same

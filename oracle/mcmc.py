@@ -425,13 +425,14 @@ def sample_hmc(target, init, step_size, num_leapfrog_steps, num_results,
 
 def sample_nuts_da(target, init_x, step_size, num_adaptation_steps,
                    num_results, num_burnin_steps, seed, target_accept_prob=0.75,
-                   max_depth=10, dtype=np.float64, groups=None, n_groups=1):
+                   max_depth=10, dtype=np.float64, groups=None, n_groups=1,
+                   problem=0):
   """sample_chain(DualAveraging(TTK(NUTS))) on an unconstrained target.
 
   `target` is already the transformed (unconstrained) target; returns the
   unconstrained states [R, C, D] and per-step results.
   """
-  draws = orng.PhiloxDraws(dtype)
+  draws = orng.PhiloxDraws(dtype, problem=problem)
   x = np.asarray(init_x, np.float64)
   c = x.shape[0]
   groups = np.zeros(c, np.int64) if groups is None else np.asarray(groups)

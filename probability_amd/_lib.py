@@ -23,7 +23,7 @@ E_INVALID, E_UNSUPPORTED, E_HIP = -1, -2, -3
 
 EXPORTED_SYMBOLS = (
     'hipmc_last_error', 'hipmc_abi_version', 'hipmc_sizeof_problem',
-    'hipmc_sizeof_run_args', 'hipmc_seed_split', 'hipmc_seed_fold_in',
+    'hipmc_sizeof_run_args', 'hipmc_sizeof_stream', 'hipmc_seed_split', 'hipmc_seed_fold_in',
     'hipmc_seed_salt', 'hipmc_philox_host', 'hipmc_hmc_run',
     'hipmc_leapfrog_run', 'hipmc_nuts_run', 'hipmc_target_eval',
     'hipmc_da_update', 'hipmc_mh_accept', 'hipmc_bijector', 'hipmc_draws')
@@ -45,7 +45,15 @@ class Problem(ctypes.Structure):
       ('bij_off', _i32 * MAX_BIJ_RANGES),
       ('bij_len', _i32 * MAX_BIJ_RANGES),
       ('bij_code', _i32 * MAX_BIJ_RANGES),
-      ('pad_', _i32),
+      ('key', ctypes.c_uint32),
+  ]
+
+
+class Stream(ctypes.Structure):
+  _fields_ = [
+      ('seed', ctypes.c_uint32 * 2), ('iter0', _i64), ('n_iters', _i32),
+      ('trace_every', _i32), ('trace', _vp), ('trace_start', _i64),
+      ('n_trace', _i32), ('pad_', _i32),
   ]
 
 
@@ -68,6 +76,7 @@ class RunArgs(ctypes.Structure):
       ('trace_start', _i64), ('trace_every', _i32), ('n_trace', _i32),
       ('out_proposed_state', _vp), ('out_proposed_tlp', _vp),
       ('out_initial_momentum', _vp), ('out_final_momentum', _vp),
+      ('streams', _vp),
   ]
 
 
@@ -98,9 +107,10 @@ def lib():
                                     _i64, _i64, _f64, _f64, _f64, _vp, _vp,
                                     _u64]
       l.hipmc_mh_accept.argtypes = [_i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp,
-                                    _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u64]
-      l.hipmc_bijector.argtypes = [_i32, _vp, _vp, _i32, _i32, _i32, _vp, _vp,
-                                   _u64]
+                                    _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                    _u64]
+      l.hipmc_bijector.argtypes = [_i32, _vp, _vp, _i32, _i32, _i32, _i32,
+                                   _vp, _vp, _u64]
       l.hipmc_draws.argtypes = [_i32, _i32, _vp, _i32, _i32, _i32, _i32, _i32,
                                 _vp, _vp, _u64]
       l.hipmc_seed_split.argtypes = [_vp, ctypes.c_uint32, _vp]
@@ -111,6 +121,8 @@ def lib():
         raise HipmcError('hipmc_problem layout mismatch')
       if l.hipmc_sizeof_run_args() != ctypes.sizeof(RunArgs):
         raise HipmcError('hipmc_run_args layout mismatch')
+      if l.hipmc_sizeof_stream() != ctypes.sizeof(Stream):
+        raise HipmcError('hipmc_stream layout mismatch')
       _lib = l
   return _lib
 

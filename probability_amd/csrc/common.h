@@ -104,6 +104,11 @@ __device__ inline double dpp_d(double v) {
   return __longlong_as_double(((int64_t)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 
+template <int Ctrl>
+__device__ inline float dpp_any(float v) { return dpp_f<Ctrl>(v); }
+template <int Ctrl>
+__device__ inline double dpp_any(double v) { return dpp_d<Ctrl>(v); }
+
 __device__ inline float readlane_f(float v, int l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
@@ -134,6 +139,21 @@ __device__ inline double wave_sum(double v) {
   v += dpp_d<kDppMirror>(v);
   return (readlane_d(v, 0) + readlane_d(v, 16)) +
          (readlane_d(v, 32) + readlane_d(v, 48));
+}
+
+// Two independent sums reduced together (interleaved for latency).
+template <typename A, typename B>
+__device__ inline void wave_sum2(A a, B b, A& ra, B& rb) {
+  a += dpp_any<kDppQuadXor1>(a);
+  b += dpp_any<kDppQuadXor1>(b);
+  a += dpp_any<kDppQuadXor2>(a);
+  b += dpp_any<kDppQuadXor2>(b);
+  a += dpp_any<kDppHalfMirror>(a);
+  b += dpp_any<kDppHalfMirror>(b);
+  a += dpp_any<kDppMirror>(a);
+  b += dpp_any<kDppMirror>(b);
+  ra = (readlane(a, 0) + readlane(a, 16)) + (readlane(a, 32) + readlane(a, 48));
+  rb = (readlane(b, 0) + readlane(b, 16)) + (readlane(b, 32) + readlane(b, 48));
 }
 
 // ---------------------------------------------------------------- scalars

@@ -17,6 +17,7 @@
 using namespace hipmc;
 
 static_assert(sizeof(hipmc_problem) == 152, "hipmc_problem layout");
+static_assert(sizeof(hipmc_stream) == 48, "hipmc_stream layout");
 static_assert(offsetof(hipmc_problem, param) == 56, "hipmc_problem layout");
 static_assert(offsetof(hipmc_problem, n_bij) == 96, "hipmc_problem layout");
 
@@ -242,13 +243,14 @@ __global__ void __launch_bounds__(256)
 
 // ------------------------------------------------------------ MH kernels
 template <typename T>
-__global__ void mh_decide_kernel(int n, Seed sd, const int32_t* chain_problem,
+__global__ void mh_decide_kernel(int n, Seed sd, const hipmc_problem* problems,
+                                 const int32_t* chain_problem,
                                  const int32_t* chain_local, const T* ptlp,
                                  const T* corr, T* tlp, T* out_lar,
                                  uint8_t* accepted) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= n) return;
-  const uint32_t prob = chain_problem ? chain_problem[c] : 0;
+  const uint32_t prob = problems[chain_problem ? chain_problem[c] : 0].key;
   const uint32_t local = chain_local ? chain_local[c] : c;
   double lar = (double)ptlp[c] - (double)tlp[c];
   if (corr) lar += (double)corr[c];
@@ -289,10 +291,11 @@ __device__ inline T softplus_inverse(T y) {
 template <typename T>
 __global__ void bijector_kernel(const hipmc_problem* problems,
                                 const int32_t* chain_problem, int n,
-                                int stride, int direction, const T* x, T* out) {
+                                int stride, int period, int direction,
+                                const T* x, T* out) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= (size_t)n * stride) return;
-  const int c = (int)(i / stride);
+  const int c = period > 0 ? (int)((i / stride) % period) : (int)(i / stride);
   const int e = (int)(i % stride);
   const hipmc_problem& pb = problems[chain_problem ? chain_problem[c] : 0];
   T v = x[i];
@@ -388,6 +391,7 @@ int hipmc_philox_host(const uint32_t ctr[4], const uint32_t key[2],
   return HIPMC_OK;
 }
 int hipmc_sizeof_run_args(void) { return (int)sizeof(hipmc_run_args); }
+int hipmc_sizeof_stream(void) { return (int)sizeof(hipmc_stream); }
 
 int hipmc_nuts_run(const hipmc_run_args* a, uint64_t stream) {
   int rc = check_run(a, true);
@@ -464,12 +468,14 @@ int hipmc_da_update(int32_t dtype, int32_t n_groups, const int32_t* step_group,
 }
 
 int hipmc_mh_accept(int32_t dtype, int32_t n, int32_t stride,
-                    const uint32_t seed[2], const int32_t* chain_problem,
+                    const uint32_t seed[2], const hipmc_problem* problems,
+                    const int32_t* chain_problem,
                     const int32_t* chain_local, const void* ps, const void* pt,
                     const void* pg, const void* corr, void* state, void* tlp,
                     void* grad, void* out_lar, uint8_t* out_acc,
                     uint64_t stream) {
-  if (!seed || !ps || !pt || !pg || !state || !tlp || !grad || !out_acc)
+  if (!seed || !problems || !ps || !pt || !pg || !state || !tlp || !grad ||
+      !out_acc)
     return fail(HIPMC_E_INVALID, "mh_accept: NULL argument");
   if (n <= 0) return HIPMC_OK;
   hipStream_t s = as_stream(stream);
@@ -477,16 +483,16 @@ int hipmc_mh_accept(int32_t dtype, int32_t n, int32_t stride,
   const size_t ne = (size_t)n * stride;
   if (dtype == HIPMC_F32) {
     hipLaunchKernelGGL(mh_decide_kernel<float>, dim3(blocks_for(n, 256)),
-                       dim3(256), 0, s, n, sd, chain_problem, chain_local,
-                       (const float*)pt, (const float*)corr, (float*)tlp,
+                       dim3(256), 0, s, n, sd, problems, chain_problem,
+                       chain_local, (const float*)pt, (const float*)corr, (float*)tlp,
                        (float*)out_lar, out_acc);
     hipLaunchKernelGGL(select_rows_kernel<float>, dim3(blocks_for(ne, 256)),
                        dim3(256), 0, s, n, stride, out_acc, (const float*)ps,
                        (const float*)pg, (float*)state, (float*)grad);
   } else if (dtype == HIPMC_F64) {
     hipLaunchKernelGGL(mh_decide_kernel<double>, dim3(blocks_for(n, 256)),
-                       dim3(256), 0, s, n, sd, chain_problem, chain_local,
-                       (const double*)pt, (const double*)corr, (double*)tlp,
+                       dim3(256), 0, s, n, sd, problems, chain_problem,
+                       chain_local, (const double*)pt, (const double*)corr, (double*)tlp,
                        (double*)out_lar, out_acc);
     hipLaunchKernelGGL(select_rows_kernel<double>, dim3(blocks_for(ne, 256)),
                        dim3(256), 0, s, n, stride, out_acc, (const double*)ps,
@@ -499,8 +505,8 @@ int hipmc_mh_accept(int32_t dtype, int32_t n, int32_t stride,
 
 int hipmc_bijector(int32_t dtype, const hipmc_problem* problems,
                    const int32_t* chain_problem, int32_t n, int32_t stride,
-                   int32_t direction, const void* x, void* out,
-                   uint64_t stream) {
+                   int32_t chain_period, int32_t direction, const void* x,
+                   void* out, uint64_t stream) {
   if (!problems || !x || !out) return fail(HIPMC_E_INVALID, "bijector: NULL argument");
   if (direction != 0 && direction != 1)
     return fail(HIPMC_E_INVALID, "bijector: direction must be 0 or 1");
@@ -510,11 +516,12 @@ int hipmc_bijector(int32_t dtype, const hipmc_problem* problems,
   if (dtype == HIPMC_F32)
     hipLaunchKernelGGL(bijector_kernel<float>, dim3(blocks_for(ne, 256)),
                        dim3(256), 0, s, problems, chain_problem, n, stride,
-                       direction, (const float*)x, (float*)out);
+                       chain_period, direction, (const float*)x, (float*)out);
   else if (dtype == HIPMC_F64)
     hipLaunchKernelGGL(bijector_kernel<double>, dim3(blocks_for(ne, 256)),
                        dim3(256), 0, s, problems, chain_problem, n, stride,
-                       direction, (const double*)x, (double*)out);
+                       chain_period, direction, (const double*)x,
+                       (double*)out);
   else
     return fail(HIPMC_E_INVALID, "bad dtype");
   return hip_check(hipGetLastError(), "bijector_kernel launch");

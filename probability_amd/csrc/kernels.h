@@ -21,16 +21,19 @@
 
 namespace hipmc {
 
-template <typename T>
-struct Run {
-  const hipmc_run_args& a;
-  __device__ const T* step_size() const { return (const T*)a.step_size; }
-};
+// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on
+// XCD b % 8).  Renumbering so that consecutive logical blocks share an XCD
+// keeps the chains of one problem -- which read the same data -- on one L2.
+__device__ inline int xcd_block(int b, int n) {
+  const int x = b & 7, per = n >> 3, rem = n & 7;
+  return x * per + (x < rem ? x : rem) + (b >> 3);
+}
 
 struct ChainId {
-  int chain;   // row in the launch
-  int prob;    // problem index
-  int local;   // chain index inside its problem (keys the draws)
+  int chain;      // row in the launch
+  int prob;       // problem index (table row)
+  int local;      // chain index inside its problem (keys the draws)
+  uint32_t key;   // problem word of the Philox counter
 };
 
 __device__ inline ChainId chain_of(const hipmc_run_args& a, int c) {
@@ -38,12 +41,79 @@ __device__ inline ChainId chain_of(const hipmc_run_args& a, int c) {
   id.chain = c;
   id.prob = a.chain_problem ? a.chain_problem[c] : 0;
   id.local = a.chain_local ? a.chain_local[c] : c;
+  id.key = a.problems[id.prob].key;
   return id;
 }
 
-__device__ inline Seed iter_seed(const hipmc_run_args& a, int it) {
-  Seed base{a.seed[0], a.seed[1]};
-  return a.fold_seed ? fold_in(base, (uint64_t)(a.iter0 + it)) : base;
+// Launch-level or per-problem sampling stream of a chain.
+struct Stream {
+  Seed base;
+  int64_t iter0;
+  int n_iters;
+  bool fold;
+  // trace
+  void* trace;
+  int64_t trace_start;
+  int trace_every, n_trace;
+  bool own_trace;  // dense per-problem trace [row][local][dim]
+};
+
+__device__ inline Stream stream_of(const hipmc_run_args& a, int prob) {
+  Stream s;
+  s.fold = a.fold_seed != 0;
+  if (a.streams) {
+    const hipmc_stream& st = a.streams[prob];
+    s.base = Seed{st.seed[0], st.seed[1]};
+    s.iter0 = st.iter0;
+    s.n_iters = st.n_iters < a.n_iters ? st.n_iters : a.n_iters;
+    s.trace = st.trace;
+    s.trace_start = st.trace_start;
+    s.trace_every = st.trace_every > 0 ? st.trace_every : 1;
+    s.n_trace = st.n_trace;
+    s.own_trace = true;
+  } else {
+    s.base = Seed{a.seed[0], a.seed[1]};
+    s.iter0 = a.iter0;
+    s.n_iters = a.n_iters;
+    s.trace = a.trace_state;
+    s.trace_start = a.trace_start;
+    s.trace_every = a.trace_every > 0 ? a.trace_every : 1;
+    s.n_trace = a.n_trace;
+    s.own_trace = false;
+  }
+  return s;
+}
+
+__device__ inline Seed iter_seed(const Stream& s, int it) {
+  return s.fold ? fold_in(s.base, (uint64_t)(s.iter0 + it)) : s.base;
+}
+
+// Trace row of launch iteration `it`, or -1 if it is not recorded.
+__device__ inline int64_t trace_row(const Stream& s, int it) {
+  if (!s.trace) return -1;
+  const int64_t g = s.iter0 + it - s.trace_start;
+  if (g < 0 || g % s.trace_every) return -1;
+  const int64_t r = g / s.trace_every;
+  return r < s.n_trace ? r : -1;
+}
+
+template <typename T, int K>
+__device__ inline void store_row(T* row, const T (&v)[K], int dim);
+
+template <typename T, int K>
+__device__ inline void trace_store(const Stream& s, const hipmc_run_args& a,
+                                   const ChainId& id, int dim, int it,
+                                   const T (&v)[K]) {
+  const int64_t tr = trace_row(s, it);
+  if (tr < 0) return;
+  if (s.own_trace) {
+    const hipmc_problem& pb = a.problems[id.prob];
+    store_row((T*)s.trace + ((size_t)tr * pb.n_chains + id.local) * dim, v,
+              dim);
+  } else {
+    store_row((T*)s.trace + ((size_t)tr * a.n_chains + id.chain) *
+                                a.dim_stride, v, dim);
+  }
 }
 
 // Standard-normal momentum of the chain: element e uses Philox block e/4,
@@ -55,8 +125,8 @@ __device__ inline void draw_momentum(Seed s, const ChainId& id, int dim,
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int e = k * 64 + lane;
-    const U4 w = philox(U4{(uint32_t)(e >> 2), (uint32_t)id.local,
-                           (uint32_t)id.prob, kPurposeMomentum},
+    const U4 w = philox(U4{(uint32_t)(e >> 2), (uint32_t)id.local, id.key,
+                           kPurposeMomentum},
                         s.k0, s.k1);
     const bool hi_pair = (e & 2) != 0;
     const uint32_t wa = hi_pair ? w.z : w.x;
@@ -142,14 +212,6 @@ __device__ inline void put(void* base, size_t i, T v) {
   if (base) ((T*)base)[i] = v;
 }
 
-// Trace row of launch iteration `it`, or -1 if it is not recorded.
-__device__ inline int64_t trace_row(const hipmc_run_args& a, int it) {
-  if (!a.trace_state) return -1;
-  const int64_t g = a.iter0 + it - a.trace_start;
-  if (g < 0 || g % a.trace_every) return -1;
-  const int64_t r = g / a.trace_every;
-  return r < a.n_trace ? r : -1;
-}
 
 // ------------------------------------------------------------------ NUTS
 // LDS per wave (elements of T): U-turn memory momentum [max_depth][K*64],
@@ -165,7 +227,7 @@ __global__ void __launch_bounds__(256) nuts_kernel(hipmc_run_args a) {
   const int lane = lane_id();
   const int wave = threadIdx.x >> 6;
   const int waves = blockDim.x >> 6;
-  const int c = blockIdx.x * waves + wave;
+  const int c = xcd_block(blockIdx.x, gridDim.x) * waves + wave;
   if (c >= a.n_chains) return;  // whole wave leaves together
   const int max_depth = a.max_tree_depth;
   T* wl = (T*)smem_raw + (size_t)wave * nuts_lds_elems<T, K>(max_depth);
@@ -188,8 +250,9 @@ __global__ void __launch_bounds__(256) nuts_kernel(hipmc_run_args a) {
   double t0 = (double)((const T*)a.tlp)[c];
   const T eps = load_step<T>(a, c);
 
-  for (int it = 0; it < a.n_iters; ++it) {
-    const Seed sd = iter_seed(a, it);
+  const Stream stream = stream_of(a, id.prob);
+  for (int it = 0; it < stream.n_iters; ++it) {
+    const Seed sd = iter_seed(stream, it);
     T pL[K], qL[K], gL[K], pR[K], qR[K], gR[K];
     draw_momentum(sd, id, dim, pL);
     const double h0 = t0 - half_sq_norm(pL);  // compute_hamiltonian, -H
@@ -212,7 +275,7 @@ __global__ void __launch_bounds__(256) nuts_kernel(hipmc_run_args a) {
     int depth = 0;
     for (; depth < max_depth && cont; ++depth) {
       // _loop_tree_doubling (nuts.py:541-711)
-      const U4 tw = philox(U4{0u, (uint32_t)id.local, (uint32_t)id.prob,
+      const U4 tw = philox(U4{0u, (uint32_t)id.local, id.key,
                               kPurposeTree | (uint32_t)depth},
                            sd.k0, sd.k1);
       const bool dir = (tw.x >> 31) != 0;
@@ -237,9 +300,45 @@ __global__ void __launch_bounds__(256) nuts_kernel(hipmc_run_args a) {
       int lf_sub = 0;
       bool csub = true, nd = notdiv;
       const int nsteps = 1 << depth;
+      // Per-lane buffers for 64 consecutive leaves: lane l holds leaf
+      // (chunk + l)'s log1p(-U) (drawn for the whole chunk at once) and its
+      // energy difference, whose exp(min(dH, 0)) terms are summed per chunk.
+      double lu_buf = 0.0, de_buf = 0.0;
+      bool de_ok = false;
       for (int i = 0; i < nsteps; ++i) {
         // _loop_build_sub_tree (nuts.py:793-946)
-        leapfrog(tg, e, nunroll, q, p, g, t);
+        const int li = i & 63;
+        if (li == 0) {
+          const U4 lw = philox(U4{(uint32_t)(i + lane), (uint32_t)id.local,
+                                  id.key, kPurposeLeaf | (uint32_t)depth},
+                               sd.k0, sd.k1);
+          lu_buf = log1p(-(double)uniform_from<T>(lw.x, lw.y));
+        }
+        double energy;
+        if (nunroll == 1) {
+          const T half = T(0.5) * e;
+#pragma unroll
+          for (int k = 0; k < K; ++k) {
+            p[k] = p[k] + half * g[k];
+            q[k] = q[k] + e * p[k];
+          }
+          double lpu;
+          const double lpl = tg.eval_partial(q, g, lpu);
+          double kin = 0.0;
+#pragma unroll
+          for (int k = 0; k < K; ++k) {
+            p[k] = p[k] + e * g[k];
+            p[k] = p[k] - half * g[k];
+            kin += (double)p[k] * (double)p[k];
+          }
+          double slp, skin;
+          wave_sum2(lpl, kin, slp, skin);
+          t = slp + lpu;
+          energy = t - 0.5 * skin;
+        } else {
+          leapfrog(tg, e, nunroll, q, p, g, t);
+          energy = t - half_sq_norm(p);
+        }
         ++lf_sub;
         if ((i & 1) == 0) {
           // store (p, cumsum before this step) in slot popcount(i)
@@ -273,32 +372,33 @@ __global__ void __launch_bounds__(256) nuts_kernel(hipmc_run_args a) {
               sl += d * mp[k * 64 + lane];
               sr += d * p[k];
             }
-            const T dl = wave_sum(sl);
-            const T dr = wave_sum(sr);
+            T dl, dr;
+            wave_sum2(sl, sr, dl, dr);
             no_u = (dl >= T(0)) && (dr >= T(0));
           }
         }
-        double energy = t - half_sq_norm(p);
         if (__builtin_isnan(energy)) energy = neg_inf();
         const double de = energy - h0;
         const bool not_div = -de < max_de;
         const double wsum = log_add_exp(wS, de);
         const double thresh = de - wsum;
-        const U4 lw = philox(U4{(uint32_t)i, (uint32_t)id.local,
-                                (uint32_t)id.prob,
-                                kPurposeLeaf | (uint32_t)depth},
-                             sd.k0, sd.k1);
-        const double u = (double)uniform_from<T>(lw.x, lw.y);
-        if (log1p(-u) <= thresh) {
+        if (readlane_d(lu_buf, li) <= thresh) {
           copy(qS, q);
           copy(gS, g);
           tS = t;
           eS = energy;
         }
         wS = wsum;
-        if (not_div) eds_sub += exp(fmin(de, 0.0));
+        if (lane == li) {
+          de_buf = de;
+          de_ok = not_div;  // energy_diff_sum counts steps that continue
+        }
         nd = nd && not_div;
         csub = not_div && no_u;
+        if (li == 63 || !csub || i + 1 == nsteps) {
+          eds_sub += wave_sum(de_ok ? exp(fmin(de_buf, 0.0)) : 0.0);
+          de_ok = false;
+        }
         if (!csub) break;
       }
       eds += eds_sub;
@@ -343,10 +443,7 @@ __global__ void __launch_bounds__(256) nuts_kernel(hipmc_run_args a) {
       put<T>(a.out_energy, o, (T)eC);
       if (a.out_tree_depth) a.out_tree_depth[o] = depth;
     }
-    const int64_t tr = trace_row(a, it);
-    if (tr >= 0)
-      store_row((T*)a.trace_state + ((size_t)tr * a.n_chains + c) * a.dim_stride,
-                q0, dim);
+    trace_store(stream, a, id, dim, it, q0);
   }
   store_row((T*)a.state + row, q0, dim);
   store_row((T*)a.grad + row, g0, dim);
@@ -366,7 +463,7 @@ __global__ void __launch_bounds__(256) hmc_kernel(hipmc_run_args a) {
   const int lane = lane_id();
   const int wave = threadIdx.x >> 6;
   const int waves = blockDim.x >> 6;
-  const int c = blockIdx.x * waves + wave;
+  const int c = xcd_block(blockIdx.x, gridDim.x) * waves + wave;
   if (c >= a.n_chains) return;
   T* scratch = (T*)smem_raw + (size_t)wave * hmc_lds_elems<T, K>();
   const ChainId id = chain_of(a, c);
@@ -383,8 +480,9 @@ __global__ void __launch_bounds__(256) hmc_kernel(hipmc_run_args a) {
   double t0 = (double)((const T*)a.tlp)[c];
   const T eps = load_step<T>(a, c);
 
-  for (int it = 0; it < a.n_iters; ++it) {
-    const Seed sd = iter_seed(a, it);
+  const Stream stream = stream_of(a, id.prob);
+  for (int it = 0; it < stream.n_iters; ++it) {
+    const Seed sd = iter_seed(stream, it);
     // MH splits the step seed into (proposal, acceptance)
     // (metropolis_hastings.py:183); the uncalibrated kernel alone uses the
     // seed it is given as its proposal seed.
@@ -407,12 +505,12 @@ __global__ void __launch_bounds__(256) hmc_kernel(hipmc_run_args a) {
       const Seed acc = split(sd, 1);
       double lar = (t - t0) + corr;  // safe_sum([t1, -t0, corr])
       if (!__builtin_isfinite(lar)) lar = neg_inf();
-      const U4 uw = philox(U4{0u, (uint32_t)id.local, (uint32_t)id.prob,
+      const U4 uw = philox(U4{0u, (uint32_t)id.local, id.key,
                               kPurposeMHAccept},
                            acc.k0, acc.k1);
       const double u = (double)uniform_from<T>(uw.x, uw.y);
       const bool accept = log(u) < lar;
-      if (it == a.n_iters - 1) {
+      if (it == stream.n_iters - 1) {
         if (a.out_proposed_state) store_row((T*)a.out_proposed_state + row, q, dim);
         if (a.out_initial_momentum) store_row((T*)a.out_initial_momentum + row, p_init, dim);
         if (a.out_final_momentum) store_row((T*)a.out_final_momentum + row, p, dim);
@@ -436,10 +534,7 @@ __global__ void __launch_bounds__(256) hmc_kernel(hipmc_run_args a) {
       if (a.out_final_momentum) store_row((T*)a.out_final_momentum + row, p, dim);
       if (lane == 0) put<T>(a.out_energy, o, (T)corr);
     }
-    const int64_t tr = trace_row(a, it);
-    if (tr >= 0)
-      store_row((T*)a.trace_state + ((size_t)tr * a.n_chains + c) * a.dim_stride,
-                q0, dim);
+    trace_store(stream, a, id, dim, it, q0);
   }
   store_row((T*)a.state + row, q0, dim);
   store_row((T*)a.grad + row, g0, dim);

@@ -38,9 +38,11 @@ struct Target {
   const T* __restrict__ y;
   // per-lane cached data in the state layout
   T a[K], b[K];
-  // bijector code of each register slot of this lane; any_bij is uniform
+  // bijector code of each register slot of this lane; the masks are
+  // wave-uniform: bit k set if some lane of slot k uses that bijector
   int code[K];
-  bool any_bij;
+  uint32_t sp_slots, ex_slots;
+  T inv_s0sq, inv_s0, inv_s1, s1sq;  // prior constants (hier)
   T* lds;  // per-wave scratch: GLM beta [K*64] followed by r [64]
 
   __device__ void load(const hipmc_problem& pb, T* wave_lds) {
@@ -55,7 +57,8 @@ struct Target {
     xt = (const T*)pb.xt;
     y = (const T*)pb.y;
     const T* aux = (const T*)pb.aux;
-    any_bij = pb.n_bij > 0;
+    sp_slots = 0u;
+    ex_slots = 0u;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int e = k * 64 + lane;
@@ -66,7 +69,13 @@ struct Target {
             e < pb.bij_off[i] + pb.bij_len[i])
           c = pb.bij_code[i];
       code[k] = c;
+      if (__ballot(c == HIPMC_BIJ_SOFTPLUS)) sp_slots |= 1u << k;
+      if (__ballot(c == HIPMC_BIJ_EXP)) ex_slots |= 1u << k;
     }
+    inv_s0 = T(1) / p0;
+    inv_s0sq = inv_s0 * inv_s0;
+    inv_s1 = T(1) / p1;
+    s1sq = p1 * p1;
     lds = wave_lds;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -87,52 +96,64 @@ struct Target {
     }
   }
 
-  // Returns the wave-uniform target log-prob (double); writes grad.
-  __device__ double eval(const T (&xs)[K], T (&g)[K]) {
+  // Per-lane partial log-prob (double) and wave-uniform part `lpu`; writes
+  // the gradient.  The caller reduces the partial (possibly together with
+  // other sums).  Bijector log-det-Jacobians are folded into the partial.
+  __device__ double eval_partial(const T (&xs)[K], T (&g)[K], double& lpu) {
     T th[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) th[k] = xs[k];
-    if (any_bij) {
-#pragma unroll
-      for (int k = 0; k < K; ++k) {
-        if (code[k] == HIPMC_BIJ_SOFTPLUS) th[k] = softplus_forward(xs[k]);
-        else if (code[k] == HIPMC_BIJ_EXP) th[k] = exp(xs[k]);
+    for (int k = 0; k < K; ++k) {
+      th[k] = xs[k];
+      if (sp_slots & (1u << k)) {
+        const T f = softplus_forward(xs[k]);
+        th[k] = code[k] == HIPMC_BIJ_SOFTPLUS ? f : th[k];
+      }
+      if (ex_slots & (1u << k)) {
+        const T f = exp(xs[k]);
+        th[k] = code[k] == HIPMC_BIJ_EXP ? f : th[k];
       }
     }
-    double lp;
+    double lpl;
     if (FAM == kFamilyHier) {
-      lp = hier_normal(th, g);
+      lpl = hier_normal(th, g, lpu);
     } else if (FAM == kFamilyGlm) {
-      lp = kind == HIPMC_TARGET_LINEAR ? glm<true>(th, g) : glm<false>(th, g);
+      lpl = kind == HIPMC_TARGET_LINEAR ? glm<true>(th, g, lpu)
+                                        : glm<false>(th, g, lpu);
     } else {
       switch (kind) {
-        case HIPMC_TARGET_DIAG_NORMAL: lp = diag_normal(th, g); break;
-        case HIPMC_TARGET_FUNNEL: lp = funnel(th, g); break;
-        default: lp = nan_exp(th, g); break;
+        case HIPMC_TARGET_DIAG_NORMAL: lpl = diag_normal(th, g, lpu); break;
+        case HIPMC_TARGET_FUNNEL: lpl = funnel(th, g, lpu); break;
+        default: lpl = nan_exp(th, g, lpu); break;
       }
     }
-    if (any_bij) {
-      double lj = 0.0;
 #pragma unroll
-      for (int k = 0; k < K; ++k) {
-        const T xv = xs[k];
-        if (code[k] == HIPMC_BIJ_SOFTPLUS) {
-          // fldj = -softplus(-x); d/dx f = sigmoid(x); d/dx fldj = sigmoid(-x)
-          lj += (double)(-softplus(-xv));
-          g[k] = g[k] * sigmoid(xv) + sigmoid(-xv);
-        } else if (code[k] == HIPMC_BIJ_EXP) {
-          lj += (double)xv;
-          g[k] = g[k] * exp(xv) + T(1);
-        }
+    for (int k = 0; k < K; ++k) {
+      const T xv = xs[k];
+      if (sp_slots & (1u << k)) {
+        // fldj = -softplus(-x); d/dx f = sigmoid(x); d/dx fldj = sigmoid(-x)
+        const bool on = code[k] == HIPMC_BIJ_SOFTPLUS;
+        lpl += on ? (double)(-softplus(-xv)) : 0.0;
+        g[k] = on ? g[k] * sigmoid(xv) + sigmoid(-xv) : g[k];
       }
-      lp += wave_sum(lj);
+      if (ex_slots & (1u << k)) {
+        const bool on = code[k] == HIPMC_BIJ_EXP;
+        lpl += on ? (double)xv : 0.0;
+        g[k] = on ? g[k] * exp(xv) + T(1) : g[k];
+      }
     }
-    return lp;
+    return lpl;
+  }
+
+  // Wave-uniform target log-prob (double); writes the gradient.
+  __device__ double eval(const T (&xs)[K], T (&g)[K]) {
+    double lpu;
+    const double lpl = eval_partial(xs, g, lpu);
+    return wave_sum(lpl) + lpu;
   }
 
   // [mu, tau, z_0..z_{J-1}], y ~ N(mu + tau z, sigma); mu ~ N(0, p0),
   // tau ~ HalfCauchy(0, p1), z ~ N(0, 1).
-  __device__ double hier_normal(const T (&th)[K], T (&g)[K]) {
+  __device__ double hier_normal(const T (&th)[K], T (&g)[K], double& lpu) {
     const int lane = lane_id();
     const T mu = readlane(th[0], 0);
     const T tau = readlane(th[0], 1);
@@ -151,22 +172,20 @@ struct Target {
       s1 += is_z ? r : T(0);
       s2 += is_z ? r * z : T(0);
     }
-    const T S1 = wave_sum(s1);
-    const T S2 = wave_sum(s2);
-    const T ms = p0, ts = p1;
-    const T gmu = -mu / (ms * ms) + S1;
-    const T gtau = T(-2) * tau / (ts * ts + tau * tau) + S2;
+    T S1, S2;
+    wave_sum2(s1, s2, S1, S2);
+    const T gmu = -mu * inv_s0sq + S1;
+    const T gtau = T(-2) * tau / (s1sq + tau * tau) + S2;
     if (lane == 0) g[0] = gmu;
     if (lane == 1) g[0] = gtau;
-    const T zm = mu / ms;
-    const T zt = tau / ts;
-    double lp = wave_sum(lpl) + log_norm + (double)(T(-0.5) * zm * zm) -
-                (double)log1p(zt * zt);
-    if (tau < T(0)) lp = neg_inf();
-    return lp;
+    const T zm = mu * inv_s0;
+    const T zt = tau * inv_s1;
+    lpu = log_norm + (double)(T(-0.5) * zm * zm) - (double)log1p(zt * zt);
+    if (tau < T(0)) lpu = neg_inf();
+    return lpl;
   }
 
-  __device__ double diag_normal(const T (&th)[K], T (&g)[K]) {
+  __device__ double diag_normal(const T (&th)[K], T (&g)[K], double& lpu) {
     const int lane = lane_id();
     double lpl = 0.0;
 #pragma unroll
@@ -177,11 +196,12 @@ struct Target {
       lpl += v ? (double)(T(-0.5) * u * u) : 0.0;
       g[k] = v ? -u * b[k] : T(0);
     }
-    return wave_sum(lpl) + log_norm;
+    lpu = log_norm;
+    return lpl;
   }
 
   // [v, x_1..x_{dim-1}]: v ~ N(0, 3), x_i ~ N(0, exp(v/2)).
-  __device__ double funnel(const T (&th)[K], T (&g)[K]) {
+  __device__ double funnel(const T (&th)[K], T (&g)[K], double& lpu) {
     const int lane = lane_id();
     const T v = readlane(th[0], 0);
     const T inv_s2 = exp(-v);
@@ -196,11 +216,12 @@ struct Target {
     const T S = wave_sum(sx2);
     const T nk = (T)(dim - 1);
     if (lane == 0) g[0] = -v / T(9) + T(0.5) * (S * inv_s2 - nk);
-    return log_norm + (double)(T(-0.5) * (v / T(3)) * (v / T(3))) -
-           (double)(T(0.5) * nk * v) - (double)(T(0.5) * S * inv_s2);
+    lpu = log_norm + (double)(T(-0.5) * (v / T(3)) * (v / T(3))) -
+          (double)(T(0.5) * nk * v) - (double)(T(0.5) * S * inv_s2);
+    return 0.0;
   }
 
-  __device__ double nan_exp(const T (&th)[K], T (&g)[K]) {
+  __device__ double nan_exp(const T (&th)[K], T (&g)[K], double& lpu) {
     const int lane = lane_id();
     double lpl = 0.0;
 #pragma unroll
@@ -212,7 +233,8 @@ struct Target {
       lpl += v ? term : 0.0;
       g[k] = (v && !(xv < T(0))) ? T(-1) : T(0);
     }
-    return wave_sum(lpl);
+    lpu = 0.0;
+    return lpl;
   }
 
   // Bayesian GLM.  LINEAR=false: logistic regression over beta (dim = D_x).
@@ -221,7 +243,7 @@ struct Target {
   // X^T r is accumulated with one state element per lane (X reads
   // coalesced), r passing through LDS 64 rows at a time.
   template <bool LINEAR>
-  __device__ double glm(const T (&th)[K], T (&g)[K]) {
+  __device__ double glm(const T (&th)[K], T (&g)[K], double& lpu) {
     const int lane = lane_id();
     const int dx = LINEAR ? dim - 1 : dim;
     T sigma = T(1), inv_s2 = T(1);
@@ -295,17 +317,15 @@ struct Target {
         xrow += dx;
       }
     }
-    const T ps = p0;
     double lp = log_norm;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int e = k * 64 + lane;
       const bool isb = e < dx;
-      const T bv = th[k];
-      lpl += isb ? (double)(T(-0.5) * (bv / ps) * (bv / ps)) : 0.0;
-      g[k] = isb ? gl[k] - bv / (ps * ps) : T(0);
+      const T bv = th[k] * inv_s0;
+      lpl += isb ? (double)(T(-0.5) * bv * bv) : 0.0;
+      g[k] = isb ? gl[k] - th[k] * inv_s0sq : T(0);
     }
-    lp += wave_sum(lpl);
     if (LINEAR) {
       const T S = wave_sum(ssq);
       const T ss = p1;
@@ -317,7 +337,8 @@ struct Target {
             (double)(T(-0.5) * (sigma / ss) * (sigma / ss));
       if (sigma < T(0)) lp = neg_inf();
     }
-    return lp;
+    lpu = lp;
+    return lpl;
   }
 };
 

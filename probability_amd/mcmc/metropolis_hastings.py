@@ -101,7 +101,8 @@ class MetropolisHastings(kernel_base.TransitionKernel):
     seed_arr = acceptance_seed.copy()
     _lib.check(_lib.lib().hipmc_mh_accept(
         _lib.dtype_code(spec.dtype), c, d,
-        seed_arr.ctypes.data_as(_lib.ctypes.c_void_p), None, None,
+        seed_arr.ctypes.data_as(_lib.ctypes.c_void_p),
+        _lib.ptr(spec.table(spec.bijectors_for_eval())), None, None,
         _lib.ptr(pstate), _lib.ptr(ptlp.contiguous()), _lib.ptr(pgrad),
         _lib.ptr(corr), _lib.ptr(state), _lib.ptr(tlp), _lib.ptr(grad),
         _lib.ptr(lar), _lib.ptr(acc), _lib.stream_handle(spec.device)))

@@ -334,7 +334,7 @@ def _run_fused(plan, num_results, current_state, num_burnin_steps, thin,
     if flat_states.numel():
       _lib.check(_lib.lib().hipmc_bijector(
           _lib.dtype_code(dt), _lib.ptr(table), None, num_results * c, d, 0,
-          _lib.ptr(flat_states), _lib.ptr(out_flat), stream))
+          0, _lib.ptr(flat_states), _lib.ptr(out_flat), stream))
   else:
     out_flat = flat_states
   rec_idx = torch.arange(num_burnin_steps, n_total, 1 + thin, device=dev)

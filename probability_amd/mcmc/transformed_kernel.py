@@ -55,7 +55,8 @@ def _map_state(spec, bijectors, parts, direction):
   out = torch.empty_like(flat)
   _lib.check(_lib.lib().hipmc_bijector(
       _lib.dtype_code(spec.dtype), _lib.ptr(spec.table(bijectors)), None,
-      flat.shape[0], flat.shape[1], direction, _lib.ptr(flat), _lib.ptr(out),
+      flat.shape[0], flat.shape[1], 0, direction, _lib.ptr(flat),
+      _lib.ptr(out),
       _lib.stream_handle(spec.device)))
   return spec.unpack_state(out, cs)
 

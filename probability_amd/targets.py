@@ -77,12 +77,16 @@ class TargetSpec:
   def _fill(self, p):
     raise NotImplementedError
 
-  def problem(self, bijectors=None, n_chains=0, chain_offset=0):
+  def problem(self, bijectors=None, n_chains=0, chain_offset=0, key=0):
+    """hipmc_problem for this target.  `key` is the problem word of the
+    Philox counter: chains with the same (seed, key, chain index) draw the
+    same numbers whatever batch they run in."""
     p = _lib.Problem()
     p.kind = self.kind
     p.dim = self.dim
     p.n_chains = n_chains
     p.chain_offset = chain_offset
+    p.key = key
     self._fill(p)
     ranges = []
     off = 0

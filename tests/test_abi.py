@@ -31,6 +31,7 @@ def test_struct_layouts():
   lib = _lib.lib()
   assert lib.hipmc_sizeof_problem() == ctypes.sizeof(_lib.Problem)
   assert lib.hipmc_sizeof_run_args() == ctypes.sizeof(_lib.RunArgs)
+  assert lib.hipmc_sizeof_stream() == ctypes.sizeof(_lib.Stream)
   assert lib.hipmc_abi_version() == 1
 
 

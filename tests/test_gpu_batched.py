@@ -1,0 +1,83 @@
+"""Batched posteriors: one launch for many problems; each problem's chains
+equal the oracle's and equal the same problem run alone."""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import mcmc as omcmc
+from oracle import rng as orng
+from oracle import targets as otg
+from probability_amd import bijectors as pbij
+from probability_amd import targets as ptg
+from probability_amd.mcmc import batched
+
+pytestmark = pytest.mark.gpu
+
+
+def _problems(dtype, n=4, seed=0):
+  r = np.random.default_rng(seed)
+  specs, bij, oracles = [], [], []
+  for k in range(n):
+    nobs = int(r.integers(40, 300))
+    d = int(r.integers(3, 90))
+    x = r.normal(size=(nobs, d)).astype(np.float32)
+    beta = r.normal(size=d) * 0.5
+    if k % 2 == 0:
+      y = (r.uniform(size=nobs) < 1 / (1 + np.exp(-(x @ beta)))).astype(
+          np.float32)
+      specs.append(ptg.LogisticRegression(x, y, dtype=dtype))
+      bij.append(None)
+      oracles.append(otg.LogisticRegression(x.astype(np.float64),
+                                            y.astype(np.float64)))
+    else:
+      y = (x @ beta + 0.7 * r.normal(size=nobs)).astype(np.float32)
+      specs.append(ptg.LinearRegression(x, y, dtype=dtype))
+      bij.append([pbij.Identity(), pbij.Exp()])
+      codes = np.zeros(d + 1, np.int32)
+      codes[d] = 2
+      oracles.append(otg.Transformed(
+          otg.LinearRegression(x.astype(np.float64), y.astype(np.float64)),
+          codes))
+  return specs, bij, oracles
+
+
+def _run(batch, seed=3):
+  return batched.sample_posteriors(batch, num_results=5, num_burnin_steps=6,
+                                   num_adaptation_steps=6, step_size=0.05,
+                                   target_accept_prob=0.8, seed=seed)
+
+
+@pytest.mark.parametrize('dtype', [torch.float32, torch.float64])
+def test_batched_equals_each_problem_alone(dtype):
+  specs, bij, _ = _problems(dtype, n=5)
+  chains = [8, 5, 8, 3, 6]
+  mix = _run(batched.PosteriorBatch(specs, chains, bij))
+  for k in range(len(specs)):
+    alone = _run(batched.PosteriorBatch([specs[k]], [chains[k]], [bij[k]],
+                                        keys=[k]))
+    np.testing.assert_array_equal(mix.draws[k].cpu().numpy(),
+                                  alone.draws[0].cpu().numpy())
+    np.testing.assert_array_equal(mix.leapfrogs_taken[k].cpu().numpy(),
+                                  alone.leapfrogs_taken[0].cpu().numpy())
+    assert float(mix.step_size[k]) == float(alone.step_size[0])
+
+
+@pytest.mark.parametrize('dtype,ndt,tol', [(torch.float32, np.float32, 2e-4),
+                                           (torch.float64, np.float64, 1e-9)])
+def test_batched_matches_oracle(dtype, ndt, tol):
+  specs, bij, oracles = _problems(dtype, n=4, seed=1)
+  chains = 6
+  res = _run(batched.PosteriorBatch(specs, chains, bij))
+  for k, ot in enumerate(oracles):
+    d = specs[k].dim
+    ox, otrace, _ = omcmc.sample_nuts_da(
+        ot, np.zeros((chains, d)), np.asarray(0.05, ndt), 6, 5, 6,
+        seed=orng.sanitize_seed(3), target_accept_prob=0.8, dtype=ndt,
+        problem=k)
+    ostates = np.stack([ot.forward(v) if hasattr(ot, 'forward') else v
+                        for v in ox])
+    lf = np.stack([r.leapfrogs_taken for r in otrace])
+    np.testing.assert_array_equal(res.leapfrogs_taken[k].cpu().numpy(), lf)
+    np.testing.assert_allclose(res.draws[k].cpu().numpy(), ostates, rtol=tol,
+                               atol=tol)
