@@ -30,6 +30,10 @@
  *                        (mcmc/hmc.py:661-729, leapfrog_integrator.py:222-362).
  *   hipmc_bijector       Bijector.forward / inverse for TTK state mapping
  *                        (bijectors/softplus.py:139-147, power_transform.py:85-100).
+ *   hipmc_glm_hmc_run    HamiltonianMonteCarlo.one_step / sample_chain for
+ *                        logistic regression at large batch: the
+ *                        autodiff gradient of the Bernoulli-logits GLM
+ *                        (bernoulli.py:119-135) as fused BF16 MFMA GEMMs.
  *   hipmc_draws          the momentum / uniform draws of samplers.normal /
  *                        samplers.uniform (internal/samplers.py:308-367) under
  *                        this library's counter-keyed Philox schedule.
@@ -194,6 +198,51 @@ typedef struct hipmc_run_args {
   const hipmc_stream* streams;
 } hipmc_run_args;
 
+/*
+ * Large-batch HMC on Bayesian logistic regression with X stored in BF16 and
+ * the all-chain gradient on MFMA (BF16 hi/lo split of positions and
+ * residuals, FP32 accumulation).  float32 state; dim must be 256; n_obs a
+ * multiple of 32; n_chains a multiple of 64.  Same momentum / MH draw
+ * schedule as hipmc_hmc_run (problem word = key, chain word = row).
+ * Workspace arrays are caller-owned (see field comments).
+ */
+typedef struct hipmc_glm_hmc_args {
+  int32_t n_obs;
+  int32_t dim;
+  int32_t n_chains;
+  int32_t n_split;          /* row splits of each gradient launch (>= 1) */
+  int32_t num_leapfrog_steps;
+  int32_t n_iters;
+  int32_t fold_seed;
+  uint32_t key;
+  int64_t iter0;
+  uint32_t seed[2];
+  double prior_scale;
+  const void* x_bf16;       /* [n_obs, dim] */
+  const void* xt_bf16;      /* [dim, n_obs] */
+  const float* y;           /* [n_obs] */
+  const float* step_size;   /* [1] or [n_chains] */
+  int32_t per_chain_step;
+  int32_t want_grad_only;   /* 1: only compute tlp/grad of `state` */
+  float* state;             /* [n_chains, dim] in/out */
+  float* tlp;               /* [n_chains] in/out */
+  float* grad;              /* [n_chains, dim] in/out */
+  float* ws_g;              /* [n_split, n_chains, dim] */
+  float* ws_ll;             /* [n_split, n_chains] */
+  float* ws_q;              /* [n_chains, dim] */
+  float* ws_p;              /* [n_chains, dim] */
+  void* ws_b;               /* [2, n_chains, dim] bf16 */
+  double* ws_ke0;           /* [n_chains] */
+  float* out_log_accept_ratio; /* [n_iters, n_chains] or NULL */
+  uint8_t* out_is_accepted;    /* [n_iters, n_chains] or NULL */
+  float* trace_state;          /* [n_trace, n_chains, dim] or NULL */
+  int64_t trace_start;
+  int32_t trace_every;
+  int32_t n_trace;
+} hipmc_glm_hmc_args;
+
+int hipmc_glm_hmc_run(const hipmc_glm_hmc_args* args, uint64_t stream);
+
 const char* hipmc_last_error(void);
 int hipmc_abi_version(void);
 
@@ -212,6 +261,7 @@ int hipmc_philox_host(const uint32_t ctr[4], const uint32_t key[2],
 int hipmc_sizeof_problem(void);
 int hipmc_sizeof_run_args(void);
 int hipmc_sizeof_stream(void);
+int hipmc_sizeof_glm_hmc_args(void);
 
 /* Fused calibrated HMC (leapfrog + MH) for n_iters iterations. */
 int hipmc_hmc_run(const hipmc_run_args* args, uint64_t stream);

@@ -23,7 +23,8 @@ E_INVALID, E_UNSUPPORTED, E_HIP = -1, -2, -3
 
 EXPORTED_SYMBOLS = (
     'hipmc_last_error', 'hipmc_abi_version', 'hipmc_sizeof_problem',
-    'hipmc_sizeof_run_args', 'hipmc_sizeof_stream', 'hipmc_seed_split', 'hipmc_seed_fold_in',
+    'hipmc_sizeof_run_args', 'hipmc_sizeof_stream',
+    'hipmc_sizeof_glm_hmc_args', 'hipmc_glm_hmc_run', 'hipmc_seed_split', 'hipmc_seed_fold_in',
     'hipmc_seed_salt', 'hipmc_philox_host', 'hipmc_hmc_run',
     'hipmc_leapfrog_run', 'hipmc_nuts_run', 'hipmc_target_eval',
     'hipmc_da_update', 'hipmc_mh_accept', 'hipmc_bijector', 'hipmc_draws')
@@ -80,6 +81,21 @@ class RunArgs(ctypes.Structure):
   ]
 
 
+class GlmHmcArgs(ctypes.Structure):
+  _fields_ = [
+      ('n_obs', _i32), ('dim', _i32), ('n_chains', _i32), ('n_split', _i32),
+      ('num_leapfrog_steps', _i32), ('n_iters', _i32), ('fold_seed', _i32),
+      ('key', ctypes.c_uint32), ('iter0', _i64), ('seed', ctypes.c_uint32 * 2),
+      ('prior_scale', _f64), ('x_bf16', _vp), ('xt_bf16', _vp), ('y', _vp),
+      ('step_size', _vp), ('per_chain_step', _i32), ('want_grad_only', _i32),
+      ('state', _vp), ('tlp', _vp), ('grad', _vp), ('ws_g', _vp),
+      ('ws_ll', _vp), ('ws_q', _vp), ('ws_p', _vp), ('ws_b', _vp),
+      ('ws_ke0', _vp), ('out_log_accept_ratio', _vp),
+      ('out_is_accepted', _vp), ('trace_state', _vp), ('trace_start', _i64),
+      ('trace_every', _i32), ('n_trace', _i32),
+  ]
+
+
 class HipmcError(RuntimeError):
   pass
 
@@ -101,6 +117,7 @@ def lib():
       l.hipmc_last_error.restype = ctypes.c_char_p
       for name in ('hipmc_hmc_run', 'hipmc_leapfrog_run', 'hipmc_nuts_run'):
         getattr(l, name).argtypes = [ctypes.POINTER(RunArgs), _u64]
+      l.hipmc_glm_hmc_run.argtypes = [ctypes.POINTER(GlmHmcArgs), _u64]
       l.hipmc_target_eval.argtypes = [_i32, _i32, _vp, _vp, _i32, _i32, _i32,
                                       _vp, _vp, _vp, _u64]
       l.hipmc_da_update.argtypes = [_i32, _i32, _vp, _vp, _vp, _i32, _i64, _vp, _vp, _vp, _vp,
@@ -123,6 +140,8 @@ def lib():
         raise HipmcError('hipmc_run_args layout mismatch')
       if l.hipmc_sizeof_stream() != ctypes.sizeof(Stream):
         raise HipmcError('hipmc_stream layout mismatch')
+      if l.hipmc_sizeof_glm_hmc_args() != ctypes.sizeof(GlmHmcArgs):
+        raise HipmcError('hipmc_glm_hmc_args layout mismatch')
       _lib = l
   return _lib
 

@@ -49,9 +49,14 @@ class PosteriorBatch:
     bijectors: per-problem list of per-part bijectors (None: identity).
     keys: per-problem Philox key (default: 0..P-1).  A problem with the same
       key, seed and chain count produces the same draws in any batch.
+    slot_dim: the kernels hold a chain's state in ceil(dim/64) registers
+      per lane, chosen from the largest dim of the launch; passing the same
+      slot_dim to every batch a problem may run in makes it run the same
+      code (bit-identical results) alone or in any mix.
   """
 
-  def __init__(self, specs, num_chains, bijectors=None, keys=None):
+  def __init__(self, specs, num_chains, bijectors=None, keys=None,
+               slot_dim=None):
     if not specs:
       raise ValueError('need at least one problem')
     self.specs = [targets_lib.as_target(s) for s in specs]
@@ -71,7 +76,7 @@ class PosteriorBatch:
     self.bijectors = bijectors or [None] * p
     self.keys = list(range(p)) if keys is None else [int(k) for k in keys]
     self.dims = [s.dim for s in self.specs]
-    self.max_dim = max(self.dims)
+    self.max_dim = max(self.dims + [int(slot_dim or 0)])
     self.dim_stride = self.max_dim
     offs = np.concatenate([[0], np.cumsum(self.num_chains)]).astype(np.int64)
     self.chain_offsets = offs

@@ -182,10 +182,15 @@ class HamiltonianMonteCarlo(kernel_base.TransitionKernel):
     return self._impl.bootstrap_results(init_state)
 
   def one_step(self, current_state, previous_kernel_results, seed=None):
-    """Fused calibrated step: one hipmc_hmc_run launch."""
+    """Fused calibrated step: one hipmc_hmc_run launch (or, for
+    LogisticRegressionBF16 at large batch, the MFMA path)."""
     inner = self._impl.inner_kernel
     spec = targets_lib.as_target(inner.target_log_prob_fn)
     acc = previous_kernel_results.accepted_results
+    if (isinstance(spec, targets_lib.LogisticRegressionBF16) and
+        spec.mfma_ok(int(acc.target_log_prob.numel()))):
+      return self._one_step_mfma(spec, current_state, previous_kernel_results,
+                                 seed)
     parts = _as_parts(current_state)
     state, tlp, grad = _runner.flat_inputs(
         spec, parts, acc.target_log_prob, _as_parts(acc.grads_target_log_prob))
@@ -230,4 +235,38 @@ class HamiltonianMonteCarlo(kernel_base.TransitionKernel):
         proposed_state=_maybe_flatten(spec.unpack_state(pst, cs),
                                       current_state),
         proposed_results=proposed_results, extra=[], seed=seed)
+    return _maybe_flatten(spec.unpack_state(state, cs), current_state), results
+
+  def mfma_runner(self, spec, n_chains):
+    key = (id(spec), n_chains)
+    r = getattr(self, '_mfma', None)
+    if r is None or r[0] != key:
+      from probability_amd.mcmc import glm_mfma
+      self._mfma = (key, glm_mfma.GlmHmcRunner(spec, n_chains))
+    return self._mfma[1]
+
+  def _one_step_mfma(self, spec, current_state, pkr, seed):
+    inner = self._impl.inner_kernel
+    acc = pkr.accepted_results
+    parts = _as_parts(current_state)
+    state, tlp, grad = _runner.flat_inputs(
+        spec, parts, acc.target_log_prob, _as_parts(acc.grads_target_log_prob))
+    step_size, nsteps = inner._step_and_steps(acc)
+    seed = samplers.sanitize_seed(seed)
+    c = state.shape[0]
+    run = self.mfma_runner(spec, c)
+    lar = torch.empty((1, c), dtype=torch.float32, device=spec.device)
+    isacc = torch.empty((1, c), dtype=torch.uint8, device=spec.device)
+    run.run(state, tlp, grad, _runner.step_tensor(step_size, spec.dtype,
+                                                  spec.device),
+            seed, nsteps, out_lar=lar, out_acc=isacc)
+    cs = tuple(acc.target_log_prob.shape)
+    accepted = isacc.reshape(cs).bool()
+    new_acc = acc._replace(target_log_prob=tlp.reshape(cs),
+                           grads_target_log_prob=spec.unpack_state(grad, cs),
+                           seed=[])
+    results = mh.MetropolisHastingsKernelResults(
+        accepted_results=new_acc, is_accepted=accepted,
+        log_accept_ratio=lar.reshape(cs), proposed_state=None,
+        proposed_results=None, extra=[], seed=seed)
     return _maybe_flatten(spec.unpack_state(state, cs), current_state), results

@@ -243,6 +243,38 @@ class LogisticRegression(TargetSpec):
     p.log_norm = d * (-_HALF_LOG_2PI - math.log(self.prior_scale))
 
 
+class LogisticRegressionBF16(LogisticRegression):
+  """Logistic regression with X stored in BF16 (large-batch HMC path).
+
+  X is rounded to BF16 once; the model is exactly the logistic regression of
+  the rounded X.  HamiltonianMonteCarlo on float32 chains with dim 256,
+  chains a multiple of 64 and n_obs a multiple of 32 runs the all-chain
+  gradient as fused MFMA GEMMs (hipmc_glm_hmc_run); other kernels use the
+  float32 copy of the rounded X.
+  """
+
+  mfma_dim = 256
+
+  def __init__(self, x, y, prior_scale=1.0, device=None):
+    dev = torch.device(device if device is not None else 'cuda')
+    xb = _to_device(x, torch.float32, dev).to(torch.bfloat16)
+    super().__init__(xb.float(), y, prior_scale, dtype=torch.float32,
+                     device=dev)
+    self.x_bf16 = xb.contiguous()
+    self.xt_bf16 = xb.t().contiguous()
+
+  def mfma_ok(self, n_chains):
+    return (self.dim == self.mfma_dim and n_chains % 64 == 0 and
+            self.n_obs % 32 == 0 and self.dtype == torch.float32)
+
+  def log_prob_and_grad_flat(self, x, bijectors=None):
+    if bijectors is None and self.mfma_ok(int(x.shape[0])):
+      from probability_amd.mcmc import glm_mfma
+      x = x.to(device=self.device, dtype=torch.float32).contiguous()
+      return glm_mfma.GlmHmcRunner(self, x.shape[0]).bootstrap(x)
+    return super().log_prob_and_grad_flat(x, bijectors)
+
+
 class LinearRegression(TargetSpec):
   """Bayesian Gaussian linear regression with unknown noise scale.
 

@@ -1,0 +1,66 @@
+"""Large-batch HMC with the BF16 MFMA contraction against the float64 oracle
+on the same BF16-rounded X.
+
+Tolerance: positions and momenta carry ~2^-17 relative error per gradient
+from the hi/lo split, so one HMC transition (L leapfrogs) is compared at
+2e-3 relative / 2e-3 absolute, log-probs at 1e-3 relative, and accept
+decisions must agree wherever the oracle's |log U - log_accept_ratio|
+exceeds 0.05.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import mcmc as omcmc
+from oracle import rng as orng
+from oracle import targets as otg
+from probability_amd import mcmc as pmcmc
+from probability_amd import targets as ptg
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(n=512, c=64, seed=0):
+  r = np.random.default_rng(seed)
+  x = (r.normal(size=(n, 256)) / 16).astype(np.float32)
+  beta = r.normal(size=256)
+  y = (r.uniform(size=n) < 1 / (1 + np.exp(-(x @ beta)))).astype(np.float32)
+  spec = ptg.LogisticRegressionBF16(x, y)
+  xr = spec.x.double().cpu().numpy()  # the BF16-rounded X
+  o = otg.LogisticRegression(xr, y.astype(np.float64))
+  q = (r.normal(size=(c, 256)) * 0.3).astype(np.float32)
+  return spec, o, q
+
+
+def test_mfma_layout_and_gradient_match_oracle():
+  spec, o, q = _case()
+  tlp, grad = spec.log_prob_and_grad_flat(torch.as_tensor(q, device='cuda'))
+  rt, rg = o.log_prob_and_grad(q.astype(np.float64))
+  np.testing.assert_allclose(tlp.cpu().numpy(), rt, rtol=1e-4)
+  np.testing.assert_allclose(grad.cpu().numpy(), rg, rtol=2e-3,
+                             atol=2e-3 * np.abs(rg).max())
+
+
+@pytest.mark.parametrize('n,c', [(512, 64), (1024, 128)])
+def test_mfma_hmc_step_matches_oracle(n, c):
+  spec, o, q = _case(n, c, seed=n)
+  kern = pmcmc.HamiltonianMonteCarlo(spec, step_size=0.05,
+                                     num_leapfrog_steps=6)
+  x = torch.as_tensor(q, device='cuda')
+  pkr = kern.bootstrap_results(x)
+  seed = orng.sanitize_seed(55)
+  nxt, res = kern.one_step(x, pkr, seed=seed)
+  t0, g0 = o.log_prob_and_grad(q.astype(np.float64))
+  oq, ores = omcmc.hmc_one_step(o, np.float32(0.05), 6, q.astype(np.float64),
+                                t0, g0, seed, orng.PhiloxDraws(np.float32))
+  u = orng.mh_uniform(orng.split_seed(seed, 2)[1], np.arange(c), np.float32)
+  margin = np.abs(np.log(u.astype(np.float64)) - ores.log_accept_ratio)
+  ok = margin > 0.05
+  got_acc = res.is_accepted.cpu().numpy()
+  np.testing.assert_array_equal(got_acc[ok], ores.is_accepted[ok])
+  np.testing.assert_allclose(res.log_accept_ratio.cpu().numpy(),
+                             ores.log_accept_ratio, atol=2e-2)
+  sel = got_acc == ores.is_accepted
+  np.testing.assert_allclose(nxt.cpu().numpy()[sel], oq[sel], rtol=2e-3,
+                             atol=2e-3)
