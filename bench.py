@@ -45,6 +45,12 @@ def parse():
   ap.add_argument('--no-cpu', action='store_true')
   ap.add_argument('--no-plain', action='store_true')
   ap.add_argument('--seed', type=int, default=20241)
+  ap.add_argument('--adapt', type=int, default=None,
+                  help='override the adaptation length (profiling runs)')
+  ap.add_argument('--draws', type=int, default=None,
+                  help='override the number of draws (profiling runs)')
+  ap.add_argument('--calib-copy', action='store_true',
+                  help='run one 256 MiB device copy (counter calibration)')
   return ap.parse_args()
 
 
@@ -140,6 +146,16 @@ def main():
   from probability_amd import workloads
 
   wl = workloads.hier_normal_j256()
+  if args.adapt is not None:
+    wl['num_adaptation_steps'] = wl['num_burnin_steps'] = args.adapt
+  if args.draws is not None:
+    wl['num_results'] = args.draws
+  if args.calib_copy:
+    src = torch.ones(64 << 20, device='cuda')
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
   j, c = len(wl['y']), wl['chains']
   spec, da, init = build_kernel(wl, torch, pa)
 

@@ -196,6 +196,9 @@ typedef struct hipmc_run_args {
   /* optional per-problem streams (device, [n_problems]); NULL: launch-level
    * seed / iter0 / trace for every problem */
   const hipmc_stream* streams;
+  /* optional row of chain c in state / tlp / grad (device, [n_chains]);
+   * NULL: row c.  Lets a launch cover a subset of a resident chain pool. */
+  const int32_t* chain_row;
 } hipmc_run_args;
 
 /*
@@ -283,7 +286,8 @@ int hipmc_target_eval(int32_t dtype, int32_t target_kind,
 /*
  * Dual-averaging update of n_steps step-size elements, applied for n_rows
  * successive DA steps (row r reads log_accept_ratio + r*row_stride and is DA
- * step `step + r`).  Element s reads the
+ * step `step + r`).  Element s keeps its DA state and step size at index
+ * state_index[s] of the state arrays (NULL: s).  Element s reads the
  * chains of group g = step_group[s] (NULL: g = s), i.e. entries
  * [group_start[g], group_start[g+1]) of `log_accept_ratio` (dtype; chains of
  * a group contiguous).  DA state arrays are float64 [n_steps]: error_sum,
@@ -293,6 +297,7 @@ int hipmc_target_eval(int32_t dtype, int32_t target_kind,
  * previous step size when set).
  */
 int hipmc_da_update(int32_t dtype, int32_t n_steps, const int32_t* step_group,
+                    const int32_t* state_index,
                     const int32_t* group_start, const void* log_accept_ratio,
                     int32_t n_rows, int64_t row_stride,
                     double* error_sum, double* log_averaging_step,

@@ -77,7 +77,7 @@ class RunArgs(ctypes.Structure):
       ('trace_start', _i64), ('trace_every', _i32), ('n_trace', _i32),
       ('out_proposed_state', _vp), ('out_proposed_tlp', _vp),
       ('out_initial_momentum', _vp), ('out_final_momentum', _vp),
-      ('streams', _vp),
+      ('streams', _vp), ('chain_row', _vp),
   ]
 
 
@@ -120,7 +120,7 @@ def lib():
       l.hipmc_glm_hmc_run.argtypes = [ctypes.POINTER(GlmHmcArgs), _u64]
       l.hipmc_target_eval.argtypes = [_i32, _i32, _vp, _vp, _i32, _i32, _i32,
                                       _vp, _vp, _vp, _u64]
-      l.hipmc_da_update.argtypes = [_i32, _i32, _vp, _vp, _vp, _i32, _i64, _vp, _vp, _vp, _vp,
+      l.hipmc_da_update.argtypes = [_i32, _i32, _vp, _vp, _vp, _vp, _i32, _i64, _vp, _vp, _vp, _vp,
                                     _i64, _i64, _f64, _f64, _f64, _vp, _vp,
                                     _u64]
       l.hipmc_mh_accept.argtypes = [_i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp,

@@ -178,15 +178,17 @@ struct EvalL {
 // (dual_averaging_step_size_adaptation.py:399-475).
 template <typename T>
 __global__ void __launch_bounds__(256)
-    da_kernel(const int32_t* step_group, const int32_t* group_start,
+    da_kernel(const int32_t* step_group, const int32_t* state_index,
+              const int32_t* group_start,
               const T* lar_base, int n_rows, int64_t row_stride,
               double* error_sum, double* log_avg, const double* log_shrink,
               const double* target_accept, int64_t step0, int64_t num_adapt,
               double gamma, double t0, double kappa, T* step_size,
               double* step_f64) {
   __shared__ double red[256];
-  const int gidx = blockIdx.x;
-  const int grp = step_group ? step_group[gidx] : gidx;
+  const int elem = blockIdx.x;
+  const int grp = step_group ? step_group[elem] : elem;
+  const int gidx = state_index ? state_index[elem] : elem;
   const int b = group_start[grp], e = group_start[grp + 1];
   const int tid = threadIdx.x;
   double err = error_sum[gidx];
@@ -491,6 +493,7 @@ int hipmc_target_eval(int32_t dtype, int32_t target_kind,
 }
 
 int hipmc_da_update(int32_t dtype, int32_t n_groups, const int32_t* step_group,
+                    const int32_t* state_index,
                     const int32_t* group_start, const void* lar,
                     int32_t n_rows, int64_t row_stride,
                     double* error_sum, double* log_averaging_step,
@@ -508,7 +511,8 @@ int hipmc_da_update(int32_t dtype, int32_t n_groups, const int32_t* step_group,
   hipStream_t s = as_stream(stream);
   if (dtype == HIPMC_F32)
     hipLaunchKernelGGL(da_kernel<float>, dim3(n_groups), dim3(256), 0, s,
-                       step_group, group_start, (const float*)lar, n_rows,
+                       step_group, state_index, group_start,
+                       (const float*)lar, n_rows,
                        row_stride, error_sum,
                        log_averaging_step, log_shrinkage_target,
                        target_accept_prob, step, num_adaptation_steps,
@@ -516,7 +520,8 @@ int hipmc_da_update(int32_t dtype, int32_t n_groups, const int32_t* step_group,
                        (float*)step_size, new_step_size_f64);
   else if (dtype == HIPMC_F64)
     hipLaunchKernelGGL(da_kernel<double>, dim3(n_groups), dim3(256), 0, s,
-                       step_group, group_start, (const double*)lar, n_rows,
+                       step_group, state_index, group_start,
+                       (const double*)lar, n_rows,
                        row_stride, error_sum,
                        log_averaging_step, log_shrinkage_target,
                        target_accept_prob, step, num_adaptation_steps,

@@ -240,14 +240,15 @@ __global__ void __launch_bounds__(256) nuts_kernel(hipmc_run_args a) {
   Target<T, K, FAM> tg;
   tg.load(pb, scratch);
   const int dim = pb.dim;
-  const size_t row = (size_t)c * a.dim_stride;
+  const int crow = a.chain_row ? a.chain_row[c] : c;
+  const size_t row = (size_t)crow * a.dim_stride;
   const double max_de = a.max_energy_diff;
   const int nunroll = a.unrolled_leapfrog_steps;
 
   T q0[K], g0[K];
   load_row(q0, (const T*)a.state + row, dim);
   load_row(g0, (const T*)a.grad + row, dim);
-  double t0 = (double)((const T*)a.tlp)[c];
+  double t0 = (double)((const T*)a.tlp)[crow];
   const T eps = load_step<T>(a, c);
 
   const Stream stream = stream_of(a, id.prob);
@@ -447,7 +448,7 @@ __global__ void __launch_bounds__(256) nuts_kernel(hipmc_run_args a) {
   }
   store_row((T*)a.state + row, q0, dim);
   store_row((T*)a.grad + row, g0, dim);
-  if (lane == 0) ((T*)a.tlp)[c] = (T)t0;
+  if (lane == 0) ((T*)a.tlp)[crow] = (T)t0;
 }
 
 // ------------------------------------------------------------------ HMC
@@ -471,13 +472,14 @@ __global__ void __launch_bounds__(256) hmc_kernel(hipmc_run_args a) {
   Target<T, K, FAM> tg;
   tg.load(pb, scratch);
   const int dim = pb.dim;
-  const size_t row = (size_t)c * a.dim_stride;
+  const int crow = a.chain_row ? a.chain_row[c] : c;
+  const size_t row = (size_t)crow * a.dim_stride;
   const int L = a.num_leapfrog_steps;
 
   T q0[K], g0[K];
   load_row(q0, (const T*)a.state + row, dim);
   load_row(g0, (const T*)a.grad + row, dim);
-  double t0 = (double)((const T*)a.tlp)[c];
+  double t0 = (double)((const T*)a.tlp)[crow];
   const T eps = load_step<T>(a, c);
 
   const Stream stream = stream_of(a, id.prob);
@@ -514,7 +516,7 @@ __global__ void __launch_bounds__(256) hmc_kernel(hipmc_run_args a) {
         if (a.out_proposed_state) store_row((T*)a.out_proposed_state + row, q, dim);
         if (a.out_initial_momentum) store_row((T*)a.out_initial_momentum + row, p_init, dim);
         if (a.out_final_momentum) store_row((T*)a.out_final_momentum + row, p, dim);
-        if (lane == 0) put<T>(a.out_proposed_tlp, c, (T)t);
+        if (lane == 0) put<T>(a.out_proposed_tlp, crow, (T)t);
       }
       if (accept) {
         copy(q0, q);
@@ -538,7 +540,7 @@ __global__ void __launch_bounds__(256) hmc_kernel(hipmc_run_args a) {
   }
   store_row((T*)a.state + row, q0, dim);
   store_row((T*)a.grad + row, g0, dim);
-  if (lane == 0) ((T*)a.tlp)[c] = (T)t0;
+  if (lane == 0) ((T*)a.tlp)[crow] = (T)t0;
 }
 
 // ------------------------------------------------------------------ eval

@@ -224,7 +224,7 @@ def sample_posteriors(batch, num_results, num_burnin_steps=0,
 
   def da(t0, n):
     _lib.check(_lib.lib().hipmc_da_update(
-        _lib.dtype_code(dt), p, None, _lib.ptr(batch.group_start),
+        _lib.dtype_code(dt), p, None, None, _lib.ptr(batch.group_start),
         _lib.ptr(lar[t0]), n, c, _lib.ptr(err), _lib.ptr(avg), _lib.ptr(lst),
         _lib.ptr(tgt), t0, num_adaptation_steps, 0.05, 10.0, 0.75,
         _lib.ptr(step), _lib.ptr(step64), stream))

@@ -204,7 +204,7 @@ class DualAveragingStepSizeAdaptation(kernel_base.TransitionKernel):
     new_step = step_size.to(device=device, dtype=dtype).reshape(-1).clone()
     step64 = step_size.to(device=device, dtype=torch.float64).reshape(-1).clone()
     _lib.check(_lib.lib().hipmc_da_update(
-        _lib.dtype_code(dtype), n, _lib.ptr(plan.step_group),
+        _lib.dtype_code(dtype), n, _lib.ptr(plan.step_group), None,
         _lib.ptr(plan.group_start), _lib.ptr(lap_flat), 1, 0, _lib.ptr(err),
         _lib.ptr(avg), _lib.ptr(lst), _lib.ptr(tgt), step_no,
         int(pkr.num_adaptation_steps), float(pkr.exploration_shrinkage),

@@ -310,6 +310,7 @@ def _run_fused(plan, num_results, current_state, num_burnin_steps, thin,
     if plan.da is not None:
       _lib.check(_lib.lib().hipmc_da_update(
           _lib.dtype_code(dt), gplan.n_steps, _lib.ptr(gplan.step_group),
+          None,
           _lib.ptr(gplan.group_start), _lib.ptr(lar_all[t]), n, c,
           _lib.ptr(err), _lib.ptr(avg), _lib.ptr(lst), _lib.ptr(tgt), t,
           n_adapt, float(pkr.exploration_shrinkage),

@@ -1,0 +1,363 @@
+"""Continuous-batching posterior sampling server.
+
+Requests (a target spec + chain count + adaptation/draw counts + seed) are
+admitted while device memory stays within an HBM budget and chain slots are
+free.  Admitted requests keep their chain state, step size and
+dual-averaging accumulators resident in device pools; every scheduling tick
+launches, per target family, one NUTS launch over all requests still adapting
+(one transition, then a per-request dual-averaging update) and one over all
+requests in their sampling phase (several transitions), on two streams so
+that the two overlap.  A request's chains are keyed by (its seed, its key,
+chain index) and advance by per-request iteration counters, so its draws
+do not depend on which other requests share the launches.
+
+Per-request semantics: sample_chain(DualAveraging(TTK(NUTS))) with
+num_burnin_steps = num_adaptation_steps (`mcmc/sample.py`, `mcmc/nuts.py`,
+`mcmc/dual_averaging_step_size_adaptation.py`).
+"""
+
+import collections
+import ctypes
+import math
+import time
+
+import numpy as np
+import torch
+
+from probability_amd import _lib
+from probability_amd import bijectors as bij_lib
+from probability_amd import random as samplers
+from probability_amd import targets as targets_lib
+
+__all__ = ['Request', 'SamplerServer']
+
+
+class Request:
+  """One posterior-sampling request."""
+
+  _next = 0
+
+  def __init__(self, spec, num_chains=16, num_adaptation_steps=500,
+               num_results=2000, seed=0, bijectors=None, step_size=0.1,
+               target_accept_prob=0.8, max_tree_depth=10, key=None,
+               arrival=None):
+    self.spec = targets_lib.as_target(spec)
+    self.num_chains = int(num_chains)
+    self.num_adaptation_steps = int(num_adaptation_steps)
+    self.num_results = int(num_results)
+    self.seed = samplers.sanitize_seed(seed, salt='mcmc.sample_chain')
+    self.bijectors = bijectors
+    self.step_size = float(step_size)
+    self.target_accept_prob = float(target_accept_prob)
+    self.max_tree_depth = int(max_tree_depth)
+    self.id = Request._next
+    Request._next += 1
+    self.key = self.id if key is None else int(key)
+    self.arrival = arrival
+    self.admitted = None
+    self.done = None
+    self.iter = 0
+    self.draws = None
+    self.final_step_size = None
+    self.rows = None
+    self.da_slot = None
+
+  @property
+  def total_steps(self):
+    return self.num_adaptation_steps + self.num_results
+
+  @property
+  def adapting(self):
+    return self.iter < self.num_adaptation_steps
+
+  def nbytes(self, dtype_size=4):
+    d = self.spec.dim
+    return (self.num_results * self.num_chains * d * dtype_size +
+            data_bytes(self.spec))
+
+
+def data_bytes(spec):
+  tot = 0
+  for name in ('x', 'xt', 'y', 'sigma', 'loc', 'scale'):
+    t = getattr(spec, name, None)
+    if isinstance(t, torch.Tensor):
+      tot += t.numel() * t.element_size()
+  return tot
+
+
+def _family(spec):
+  return 'hier' if spec.kind == _lib.TARGET_HIER_NORMAL else 'glm'
+
+
+class _Pool:
+  """Device-resident chain and dual-averaging state of one target family."""
+
+  def __init__(self, family, slot_dim, max_chains, max_requests, dtype,
+               device):
+    self.family = family
+    self.dim_stride = int(slot_dim)
+    self.dtype, self.device = dtype, device
+    self.state = torch.zeros((max_chains, slot_dim), dtype=dtype, device=device)
+    self.tlp = torch.zeros(max_chains, dtype=dtype, device=device)
+    self.grad = torch.zeros((max_chains, slot_dim), dtype=dtype, device=device)
+    f64 = torch.float64
+    self.step = torch.zeros(max_requests, dtype=dtype, device=device)
+    self.step64 = torch.zeros(max_requests, dtype=f64, device=device)
+    self.err = torch.zeros(max_requests, dtype=f64, device=device)
+    self.avg = torch.zeros(max_requests, dtype=f64, device=device)
+    self.lst = torch.zeros(max_requests, dtype=f64, device=device)
+    self.tgt = torch.zeros(max_requests, dtype=f64, device=device)
+    self.free_rows = list(range(max_chains))
+    self.free_da = list(range(max_requests))
+
+  def can_hold(self, req):
+    return (len(self.free_rows) >= req.num_chains and self.free_da and
+            req.spec.dim <= self.dim_stride)
+
+  def admit(self, req):
+    req.rows = [self.free_rows.pop(0) for _ in range(req.num_chains)]
+    req.da_slot = self.free_da.pop(0)
+    s = req.da_slot
+    self.step[s] = req.step_size
+    self.step64[s] = float(np.float64(np.asarray(req.step_size,
+                                                 np.float32 if self.dtype ==
+                                                 torch.float32 else
+                                                 np.float64)))
+    self.err[s] = 0.0
+    self.avg[s] = 0.0
+    self.lst[s] = math.log(10.0) + math.log(float(self.step64[s]))
+    self.tgt[s] = req.target_accept_prob
+
+  def release(self, req):
+    self.free_rows.extend(req.rows)
+    self.free_rows.sort()
+    self.free_da.append(req.da_slot)
+    req.rows = None
+
+
+class SamplerServer:
+  """Admits requests by HBM budget and advances them in shared launches.
+
+  Args:
+    hbm_budget_bytes: device bytes the server may hold for admitted requests
+      (their data and draw buffers).
+    iters_per_tick: transitions a sampling-phase request makes per launch.
+    slot_dims: register-slot class per family ({'glm': 192, 'hier': 512});
+      every request of a family runs the same kernel code.
+  """
+
+  def __init__(self, dtype=torch.float32, device='cuda',
+               hbm_budget_bytes=32 << 30, max_chains=32768, max_requests=2048,
+               iters_per_tick=8, slot_dims=None):
+    _lib.require_cuda()
+    self.dtype = dtype
+    self.device = torch.device(device)
+    self.budget = int(hbm_budget_bytes)
+    self.used = 0
+    self.iters_per_tick = int(iters_per_tick)
+    slot_dims = slot_dims or {'glm': 192, 'hier': 512}
+    self.pools = {f: _Pool(f, slot_dims[f], max_chains, max_requests, dtype,
+                           self.device) for f in slot_dims}
+    self.queue = collections.deque()
+    self.active = []
+    self.finished = []
+    self.streams = [torch.cuda.Stream(device=self.device),
+                    torch.cuda.Stream(device=self.device)]
+    self.ticks = 0
+
+  # ------------------------------------------------------------ admission
+  def submit(self, req):
+    if req.arrival is None:
+      req.arrival = time.perf_counter()
+    self.queue.append(req)
+
+  def _admit(self):
+    admitted = []
+    while self.queue:
+      req = self.queue[0]
+      pool = self.pools[_family(req.spec)]
+      need = req.nbytes(torch.tensor([], dtype=self.dtype).element_size())
+      if self.used + need > self.budget or not pool.can_hold(req):
+        break
+      self.queue.popleft()
+      self.used += need
+      pool.admit(req)
+      req.admitted = time.perf_counter()
+      req.draws = torch.empty((req.num_results, req.num_chains, req.spec.dim),
+                              dtype=self.dtype, device=self.device)
+      tspec = (targets_lib.TransformedTarget(req.spec, req.bijectors)
+               if req.bijectors else req.spec)
+      req.table = tspec.table(tspec.bijectors_for_eval())
+      req.problem_bytes = bytes(tspec.problem(tspec.bijectors_for_eval()))
+      x = torch.zeros((req.num_chains, req.spec.dim), dtype=self.dtype,
+                      device=self.device)
+      tlp, grad = tspec.log_prob_and_grad_flat(x)
+      rows = torch.as_tensor(req.rows, device=self.device)
+      pool.state[rows] = torch.nn.functional.pad(
+          x, (0, pool.dim_stride - x.shape[1]))
+      pool.grad[rows] = torch.nn.functional.pad(
+          grad, (0, pool.dim_stride - grad.shape[1]))
+      pool.tlp[rows] = tlp
+      self.active.append(req)
+      admitted.append(req)
+    return admitted
+
+  # ------------------------------------------------------------ launches
+  def _launch(self, pool, reqs, adapting, stream):
+    if not reqs:
+      return None
+    dt = self.dtype
+    dev = self.device
+    p = len(reqs)
+    raw = bytearray()
+    cp, cl, rows, cs = [], [], [], []
+    sarr = (_lib.Stream * p)()
+    offs = [0]
+    for i, r in enumerate(reqs):
+      prob = _lib.Problem.from_buffer_copy(r.problem_bytes)
+      prob.n_chains = r.num_chains
+      prob.chain_offset = offs[-1]
+      prob.key = r.key
+      raw += bytes(prob)
+      cp += [i] * r.num_chains
+      cl += list(range(r.num_chains))
+      rows += r.rows
+      cs += [r.da_slot] * r.num_chains
+      offs.append(offs[-1] + r.num_chains)
+      st = sarr[i]
+      st.seed[0], st.seed[1] = int(r.seed[0]), int(r.seed[1])
+      st.iter0 = r.iter
+      st.n_iters = 1 if adapting else min(self.iters_per_tick,
+                                          r.total_steps - r.iter)
+      st.trace_every = 1
+      st.trace = r.draws.data_ptr()
+      st.trace_start = r.num_adaptation_steps
+      st.n_trace = r.num_results
+    n = offs[-1]
+    n_iters = 1 if adapting else self.iters_per_tick
+    with torch.cuda.stream(stream):
+      table = torch.frombuffer(raw, dtype=torch.uint8).to(dev, non_blocking=True)
+      streams_t = torch.frombuffer(bytearray(bytes(sarr)), dtype=torch.uint8
+                                   ).to(dev, non_blocking=True)
+      ints = torch.as_tensor(np.array([cp, cl, rows, cs], np.int32)).to(
+          dev, non_blocking=True)
+      lar = torch.empty((n_iters, n), dtype=dt, device=dev)
+      a = _lib.RunArgs()
+      a.dtype = _lib.dtype_code(dt)
+      a.n_chains = n
+      a.dim_stride = pool.dim_stride
+      a.max_dim = pool.dim_stride
+      a.n_iters = n_iters
+      a.fold_seed = 1
+      a.max_tree_depth = reqs[0].max_tree_depth
+      a.unrolled_leapfrog_steps = 1
+      a.target_kind = (_lib.TARGET_HIER_NORMAL if pool.family == 'hier'
+                       else _lib.TARGET_LOGISTIC)
+      a.max_energy_diff = 1000.0
+      a.problems = table.data_ptr()
+      a.n_problems = p
+      a.chain_problem = ints[0].data_ptr()
+      a.chain_local = ints[1].data_ptr()
+      a.chain_row = ints[2].data_ptr()
+      a.chain_step = ints[3].data_ptr()
+      a.step_size = pool.step.data_ptr()
+      a.state, a.tlp, a.grad = (pool.state.data_ptr(), pool.tlp.data_ptr(),
+                                pool.grad.data_ptr())
+      a.out_log_accept_ratio = lar.data_ptr()
+      a.streams = streams_t.data_ptr()
+      _lib.check(_lib.lib().hipmc_nuts_run(a, stream.cuda_stream))
+      if adapting:
+        gstart = torch.as_tensor(np.asarray(offs, np.int32)).to(
+            dev, non_blocking=True)
+        slots = torch.as_tensor(np.asarray([r.da_slot for r in reqs],
+                                           np.int32)).to(dev, non_blocking=True)
+        key0 = (reqs[0].iter, reqs[0].num_adaptation_steps)
+        if any((r.iter, r.num_adaptation_steps) != key0 for r in reqs):
+          # requests at different DA steps: one update per distinct step
+          groups = collections.defaultdict(list)
+          for i, r in enumerate(reqs):
+            groups[(r.iter, r.num_adaptation_steps)].append(i)
+          for (step_no, _), idx in groups.items():
+            sub = torch.as_tensor(np.asarray(idx, np.int32)).to(
+                dev, non_blocking=True)
+            self._da(pool, reqs[idx[0]], sub, slots, gstart, lar, n,
+                     step_no, stream)
+        else:
+          self._da(pool, reqs[0], None, slots, gstart, lar, n, key0[0],
+                   stream)
+      keep = (table, streams_t, ints, lar)
+    return keep
+
+  def _da(self, pool, req, elems, slots, gstart, lar, n, step_no, stream):
+    if elems is None:
+      n_el = slots.numel()
+      sg, si = None, slots
+    else:
+      n_el = elems.numel()
+      sg, si = elems, slots.index_select(0, elems.long()).to(torch.int32)
+    _lib.check(_lib.lib().hipmc_da_update(
+        _lib.dtype_code(self.dtype), n_el, _lib.ptr(sg), _lib.ptr(si),
+        _lib.ptr(gstart), _lib.ptr(lar), 1, n, _lib.ptr(pool.err),
+        _lib.ptr(pool.avg), _lib.ptr(pool.lst), _lib.ptr(pool.tgt), step_no,
+        req.num_adaptation_steps, 0.05, 10.0, 0.75, _lib.ptr(pool.step),
+        _lib.ptr(pool.step64), stream.cuda_stream))
+
+  def tick(self):
+    """Admits what fits, advances every active request once, retires the
+    finished ones.  Returns the requests finished in this tick."""
+    self._admit()
+    keep = []
+    for fam, pool in self.pools.items():
+      reqs = [r for r in self.active if _family(r.spec) == fam]
+      ad = [r for r in reqs if r.adapting]
+      sm = [r for r in reqs if not r.adapting]
+      keep.append(self._launch(pool, ad, True, self.streams[0]))
+      keep.append(self._launch(pool, sm, False, self.streams[1]))
+    for s in self.streams:
+      s.synchronize()
+    del keep
+    self.ticks += 1
+    done = []
+    now = time.perf_counter()
+    for r in self.active:
+      if r.adapting:
+        r.iter += 1
+      else:
+        r.iter = min(r.total_steps, r.iter + self.iters_per_tick)
+      if r.iter >= r.total_steps:
+        done.append(r)
+    for r in done:
+      pool = self.pools[_family(r.spec)]
+      r.final_step_size = float(pool.step[r.da_slot])
+      pool.release(r)
+      self.active.remove(r)
+      self.used -= r.nbytes(torch.tensor([], dtype=self.dtype).element_size())
+      r.done = now
+      if r.bijectors:
+        r.draws = _forward(r, self.dtype)
+      self.finished.append(r)
+    return done
+
+  def idle(self):
+    return not self.queue and not self.active
+
+
+def _forward(req, dtype):
+  """Constrains a finished request's draws (TTK forward map)."""
+  d = req.spec.dim
+  flat = req.draws.reshape(-1, d)
+  out = torch.empty_like(flat)
+  _lib.check(_lib.lib().hipmc_bijector(
+      _lib.dtype_code(dtype), _lib.ptr(req.table), None, flat.shape[0], d, 0,
+      0, _lib.ptr(flat), _lib.ptr(out), _lib.stream_handle(flat.device)))
+  return out.reshape(req.draws.shape)
+
+
+def run_alone(req, dtype=torch.float32, device='cuda', iters_per_tick=8):
+  """Runs one request by itself through the same server (reference for the
+  alone-vs-mix equality)."""
+  srv = SamplerServer(dtype=dtype, device=device, iters_per_tick=iters_per_tick)
+  srv.submit(req)
+  while not srv.idle():
+    srv.tick()
+  return req
