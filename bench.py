@@ -196,6 +196,12 @@ def main():
   flops = np.array(flops)
   achieved = float(flops.sum() / (kern_ms.sum() * 1e-3) / 1e12)
   adapt_ms = kern_ms[:-1] if len(kern_ms) > 1 else kern_ms
+  # latency of one leapfrog on the critical path of an adaptation launch
+  # (one transition per launch: the launch lasts as long as its longest
+  # trajectory)
+  lf_max = stats['leapfrogs_max_per_step'].double().cpu().numpy()
+  crit_us = [1000.0 * ms_k / max(lf_max[t0_], 1.0)
+             for (ms_k, t0_, n_) in launches if n_ == 1]
   grad_evals = float(stats['leapfrogs_total'])
 
   line = {
@@ -227,6 +233,10 @@ def main():
       'final_step_size': float(res.final_kernel_results.new_step_size),
       'adapt_iteration_ms_p50_p99': [float(np.percentile(adapt_ms, 50)),
                                      float(np.percentile(adapt_ms, 99))],
+      'adapt_longest_trajectory_p50': float(np.percentile(
+          lf_max[:len(crit_us)], 50)) if crit_us else None,
+      'adapt_us_per_critical_leapfrog_p50': (
+          float(np.percentile(crit_us, 50)) if crit_us else None),
       'roofline': {
           'bound': 'mfma', 'achieved': achieved, 'peak': FP32_PEAK_TFLOPS,
           'unit': 'TFLOP/s', 'frac': achieved / FP32_PEAK_TFLOPS,

@@ -89,26 +89,6 @@ constexpr int kDppQuadXor2 = 0x4E;    // quad_perm [2,3,0,1]
 constexpr int kDppHalfMirror = 0x141;  // lane i <-> 7-i within 8
 constexpr int kDppMirror = 0x140;      // lane i <-> 15-i within 16
 
-template <int Ctrl>
-__device__ inline float dpp_f(float v) {
-  return __int_as_float(
-      __builtin_amdgcn_mov_dpp(__float_as_int(v), Ctrl, 0xF, 0xF, false));
-}
-template <int Ctrl>
-__device__ inline double dpp_d(double v) {
-  const int64_t b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)b, Ctrl, 0xF, 0xF,
-                                          false);
-  const int hi = __builtin_amdgcn_mov_dpp((int)(uint32_t)(b >> 32), Ctrl, 0xF,
-                                          0xF, false);
-  return __longlong_as_double(((int64_t)(uint32_t)hi << 32) | (uint32_t)lo);
-}
-
-template <int Ctrl>
-__device__ inline float dpp_any(float v) { return dpp_f<Ctrl>(v); }
-template <int Ctrl>
-__device__ inline double dpp_any(double v) { return dpp_d<Ctrl>(v); }
-
 __device__ inline float readlane_f(float v, int l) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
@@ -121,50 +101,157 @@ __device__ inline double readlane_d(double v, int l) {
 __device__ inline float readlane(float v, int l) { return readlane_f(v, l); }
 __device__ inline double readlane(double v, int l) { return readlane_d(v, l); }
 
+// v + v[perm(lane)] for a DPP permutation inside each row of 16 lanes.  The
+// float form compiles to one v_add_f32_dpp.
+template <int Ctrl>
+__device__ inline float dpp_add(float v) {
+  return v + __int_as_float(__builtin_amdgcn_update_dpp(
+                 0, __float_as_int(v), Ctrl, 0xF, 0xF, true));
+}
+template <int Ctrl>
+__device__ inline double dpp_add(double v) {
+  const int64_t b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, Ctrl, 0xF,
+                                             0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), Ctrl,
+                                             0xF, 0xF, true);
+  return v + __longlong_as_double(((int64_t)(uint32_t)hi << 32) |
+                                  (uint32_t)lo);
+}
+// Row broadcast (row_bcast:15 / row_bcast:31) into the rows of RowMask; the
+// other rows add zero.
+template <int Ctrl, int RowMask>
+__device__ inline float dpp_add_rows(float v) {
+  return v + __int_as_float(__builtin_amdgcn_update_dpp(
+                 0, __float_as_int(v), Ctrl, RowMask, 0xF, false));
+}
+template <int Ctrl, int RowMask>
+__device__ inline double dpp_add_rows(double v) {
+  const int64_t b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, Ctrl,
+                                             RowMask, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), Ctrl,
+                                             RowMask, 0xF, false);
+  return v + __longlong_as_double(((int64_t)(uint32_t)hi << 32) |
+                                  (uint32_t)lo);
+}
+
+constexpr int kDppRowBcast15 = 0x142;
+constexpr int kDppRowBcast31 = 0x143;
+
+// Row sums: after the four butterfly stages every lane of a row of 16 holds
+// the same row sum (each add is commutative, so all lanes round alike).
+template <typename A>
+__device__ inline A row_sum(A v) {
+  v = dpp_add<kDppQuadXor1>(v);
+  v = dpp_add<kDppQuadXor2>(v);
+  v = dpp_add<kDppHalfMirror>(v);
+  return dpp_add<kDppMirror>(v);
+}
+// Combines row sums r0..r3: rows 1 and 3 take r1 + r0 and r3 + r2, then row 3
+// takes (r3 + r2) + (r1 + r0), read from lane 63.  The result equals
+// (r0 + r1) + (r2 + r3) bit for bit (commutative adds).
+template <typename A>
+__device__ inline A rows_total(A v) {
+  v = dpp_add_rows<kDppRowBcast15, 0xA>(v);
+  v = dpp_add_rows<kDppRowBcast31, 0xC>(v);
+  return readlane(v, 63);
+}
+
 // Sum over all 64 lanes; every lane receives the (wave-uniform) result.
-// Requires a full EXEC mask.  Rows of 16 are reduced with DPP, the four row
-// sums are combined through readlane in a fixed order (deterministic).
-__device__ inline float wave_sum(float v) {
-  v += dpp_f<kDppQuadXor1>(v);
-  v += dpp_f<kDppQuadXor2>(v);
-  v += dpp_f<kDppHalfMirror>(v);
-  v += dpp_f<kDppMirror>(v);
-  return (readlane_f(v, 0) + readlane_f(v, 16)) +
-         (readlane_f(v, 32) + readlane_f(v, 48));
-}
-__device__ inline double wave_sum(double v) {
-  v += dpp_d<kDppQuadXor1>(v);
-  v += dpp_d<kDppQuadXor2>(v);
-  v += dpp_d<kDppHalfMirror>(v);
-  v += dpp_d<kDppMirror>(v);
-  return (readlane_d(v, 0) + readlane_d(v, 16)) +
-         (readlane_d(v, 32) + readlane_d(v, 48));
-}
+// Requires a full EXEC mask.  Deterministic: fixed reduction order.
+__device__ inline float wave_sum(float v) { return rows_total(row_sum(v)); }
+__device__ inline double wave_sum(double v) { return rows_total(row_sum(v)); }
 
 // Two independent sums reduced together (interleaved for latency).
 template <typename A, typename B>
 __device__ inline void wave_sum2(A a, B b, A& ra, B& rb) {
-  a += dpp_any<kDppQuadXor1>(a);
-  b += dpp_any<kDppQuadXor1>(b);
-  a += dpp_any<kDppQuadXor2>(a);
-  b += dpp_any<kDppQuadXor2>(b);
-  a += dpp_any<kDppHalfMirror>(a);
-  b += dpp_any<kDppHalfMirror>(b);
-  a += dpp_any<kDppMirror>(a);
-  b += dpp_any<kDppMirror>(b);
-  ra = (readlane(a, 0) + readlane(a, 16)) + (readlane(a, 32) + readlane(a, 48));
-  rb = (readlane(b, 0) + readlane(b, 16)) + (readlane(b, 32) + readlane(b, 48));
+  a = dpp_add<kDppQuadXor1>(a);
+  b = dpp_add<kDppQuadXor1>(b);
+  a = dpp_add<kDppQuadXor2>(a);
+  b = dpp_add<kDppQuadXor2>(b);
+  a = dpp_add<kDppHalfMirror>(a);
+  b = dpp_add<kDppHalfMirror>(b);
+  a = dpp_add<kDppMirror>(a);
+  b = dpp_add<kDppMirror>(b);
+  a = dpp_add_rows<kDppRowBcast15, 0xA>(a);
+  b = dpp_add_rows<kDppRowBcast15, 0xA>(b);
+  a = dpp_add_rows<kDppRowBcast31, 0xC>(a);
+  b = dpp_add_rows<kDppRowBcast31, 0xC>(b);
+  ra = readlane(a, 63);
+  rb = readlane(b, 63);
 }
 
 // ---------------------------------------------------------------- scalars
 __device__ inline double neg_inf() { return -__builtin_inf(); }
 
+// An empty volatile asm: first in a wave-uniform branch whose body is
+// rarely taken, it keeps the compiler from if-converting (speculating) the
+// body into every execution.
+__device__ inline void keep_branch() { asm volatile(""); }
+
+// 1 / d for normal, finite d: hardware reciprocal plus one Newton step
+// (float), IEEE division (double).
+__device__ inline float recip(float d) {
+  const float r = __builtin_amdgcn_rcpf(d);
+  return fmaf(fmaf(-d, r, 1.0f), r, r);
+}
+__device__ inline double recip(double d) { return 1.0 / d; }
+
+// log1p(-u) for a uniform u in [0, 1): 1 - u is exact for the 24-bit float
+// uniforms, so log(1 - u) (float) is as accurate as the log; libm (double).
+__device__ inline float log1m(float u) { return logf(1.0f - u); }
+__device__ inline double log1m(double u) { return log1p(-u); }
+
+// 1 / (1 + e) for e in [0, 1]: hardware reciprocal plus one Newton step
+// (float), IEEE division (double).
+__device__ inline float recip_1p(float e) {
+  const float d = 1.0f + e;
+  const float r = __builtin_amdgcn_rcpf(d);
+  return fmaf(fmaf(-d, r, 1.0f), r, r);
+}
+__device__ inline double recip_1p(double e) { return 1.0 / (1.0 + e); }
+
+// log1p(u) for u in [0, 1].  Float: log1p(u) = 2 atanh(s) with
+// s = u / (2 + u) in [0, 1/3] (one Newton correction after the hardware
+// reciprocal) and the odd series summed to s^15 (truncation below 2^-27
+// relative); within 2 ulp of the correctly rounded value on [0, 1] in a
+// float32 emulation over 3e5 points.  NaN propagates.  Double: libm.
+__device__ inline float log1p_unit(float u) {
+  const float d = 2.0f + u;
+  const float r = __builtin_amdgcn_rcpf(d);
+  float s = u * r;
+  s = fmaf(fmaf(-s, d, u), r, s);
+  const float s2 = s * s;
+  float q = 1.0f / 15.0f;
+  q = fmaf(q, s2, 1.0f / 13.0f);
+  q = fmaf(q, s2, 1.0f / 11.0f);
+  q = fmaf(q, s2, 1.0f / 9.0f);
+  q = fmaf(q, s2, 1.0f / 7.0f);
+  q = fmaf(q, s2, 1.0f / 5.0f);
+  q = fmaf(q, s2, 1.0f / 3.0f);
+  const float t = s2 * q;
+  const float s2x = 2.0f * s;
+  return fmaf(s2x, t, s2x);
+}
+__device__ inline double log1p_unit(double u) { return log1p(u); }
+
+// log1p(v) for v >= 0: the series above on [0, 1], log(1 + v) beyond
+// (where rounding 1 + v costs under one ulp of the result).
+__device__ inline float log1p_pos(float v) {
+  return v <= 1.0f ? log1p_unit(v) : logf(1.0f + v);
+}
+__device__ inline double log1p_pos(double v) { return log1p(v); }
+
 // log(exp(x) + exp(y)) as `math/generic.py:605-611`:
-// larger + softplus((x - larger) + (y - larger)).
+// larger + softplus((x - larger) + (y - larger)); the argument is -|x - y|
+// (or NaN), so the softplus is log1p(exp(.)) of a value in [0, 1], taken
+// at precision P (the chain's state dtype) around double endpoints.
+template <typename P>
 __device__ inline double log_add_exp(double x, double y) {
   const double larger = fmax(x, y);
   const double z = (x - larger) + (y - larger);
-  return larger + (log1p(exp(-fabs(z))) + fmax(z, 0.0));
+  return larger + (double)log1p_unit(exp((P)z));
 }
 
 template <typename T>
@@ -176,6 +263,24 @@ __device__ inline T sigmoid(T x) {
   const T e = exp(-fabs(x));
   return x >= T(0) ? T(1) / (T(1) + e) : e / (T(1) + e);
 }
+
+// softplus(+-x) and sigmoid(+-x) from one exp, one log1p and one
+// reciprocal: with e = exp(-|x|) and l = log1p(e),
+//   softplus(x) = max(x, 0) + l,      softplus(-x) = max(-x, 0) + l,
+//   sigmoid(x) = 1/(1+e) or e/(1+e) (x >= 0 or not), sigmoid(-x) the other.
+template <typename T>
+struct SoftplusParts {
+  T l, sig, sig_neg;
+};
+template <typename T>
+__device__ inline SoftplusParts<T> softplus_parts(T x) {
+  const T e = exp(-fabs(x));
+  const T r = recip_1p(e);
+  const T er = e * r;
+  const bool pos = x >= T(0);
+  return SoftplusParts<T>{log1p_unit(e), pos ? r : er, pos ? er : r};
+}
+
 // Softplus bijector forward (`bijectors/softplus.py:36-50`): log1p(exp(x))
 // below -9 (float) / -20 (double), else softplus.
 template <typename T>

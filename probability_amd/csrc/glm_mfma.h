@@ -108,12 +108,12 @@ __global__ void __launch_bounds__(256, 2)
         const float sv = mt ? s1[i] : s0[i];
         const float yv = ys[row];
         const float e = __expf(-fabsf(sv));
-        const float inv = 1.f / (1.f + e);
+        const float inv = recip_1p(e);
         const float sig = sv >= 0.f ? inv : e * inv;
         const float r = yv - sig;
         if (want_ll) {
           // multiply_no_nan(-softplus(s), 1-y) + multiply_no_nan(-softplus(-s), y)
-          const float sp_abs = log1pf(e);
+          const float sp_abs = log1p_unit(e);
           const float sp_pos = sp_abs + fmaxf(sv, 0.f);   // softplus(s)
           const float sp_neg = sp_abs + fmaxf(-sv, 0.f);  // softplus(-s)
           const float w0 = 1.f - yv;

@@ -280,7 +280,7 @@ __global__ void __launch_bounds__(256) nuts_kernel(hipmc_run_args a) {
                               kPurposeTree | (uint32_t)depth},
                            sd.k0, sd.k1);
       const bool dir = (tw.x >> 31) != 0;
-      const double tree_u = (double)uniform_from<T>(tw.y, tw.z);
+      const T tree_u = uniform_from<T>(tw.y, tw.z);
       const T e = dir ? eps : -eps;
       T q[K], p[K], g[K];
       double t;
@@ -313,7 +313,7 @@ __global__ void __launch_bounds__(256) nuts_kernel(hipmc_run_args a) {
           const U4 lw = philox(U4{(uint32_t)(i + lane), (uint32_t)id.local,
                                   id.key, kPurposeLeaf | (uint32_t)depth},
                                sd.k0, sd.k1);
-          lu_buf = log1p(-(double)uniform_from<T>(lw.x, lw.y));
+          lu_buf = (double)log1m(uniform_from<T>(lw.x, lw.y));
         }
         double energy;
         if (nunroll == 1) {
@@ -381,7 +381,7 @@ __global__ void __launch_bounds__(256) nuts_kernel(hipmc_run_args a) {
         if (__builtin_isnan(energy)) energy = neg_inf();
         const double de = energy - h0;
         const bool not_div = -de < max_de;
-        const double wsum = log_add_exp(wS, de);
+        const double wsum = log_add_exp<T>(wS, de);
         const double thresh = de - wsum;
         if (readlane_d(lu_buf, li) <= thresh) {
           copy(qS, q);
@@ -405,10 +405,10 @@ __global__ void __launch_bounds__(256) nuts_kernel(hipmc_run_args a) {
       eds += eds_sub;
       lf += lf_sub;
       const double tree_w = csub ? wS : neg_inf();
-      const double wsum = log_add_exp(tree_w, wC);
+      const double wsum = log_add_exp<T>(tree_w, wC);
       double thresh = tree_w - wC;
       if (__builtin_isnan(thresh)) thresh = 0.0;
-      const bool choose = (log1p(-tree_u) <= thresh) && csub;
+      const bool choose = ((double)log1m(tree_u) <= thresh) && csub;
       if (choose) {
         copy(qC, qS);
         copy(gC, gS);

@@ -101,16 +101,23 @@ struct Target {
   // other sums).  Bijector log-det-Jacobians are folded into the partial.
   __device__ double eval_partial(const T (&xs)[K], T (&g)[K], double& lpu) {
     T th[K];
+    SoftplusParts<T> sp[K];
+    T ex[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       th[k] = xs[k];
       if (sp_slots & (1u << k)) {
-        const T f = softplus_forward(xs[k]);
+        keep_branch();
+        // Softplus bijector forward (bijectors/softplus.py:36-50): below the
+        // cutoff TFP takes log1p(exp(x)), which equals max(x, 0) + l there.
+        sp[k] = softplus_parts(xs[k]);
+        const T f = fmax(xs[k], T(0)) + sp[k].l;
         th[k] = code[k] == HIPMC_BIJ_SOFTPLUS ? f : th[k];
       }
       if (ex_slots & (1u << k)) {
-        const T f = exp(xs[k]);
-        th[k] = code[k] == HIPMC_BIJ_EXP ? f : th[k];
+        keep_branch();
+        ex[k] = exp(xs[k]);
+        th[k] = code[k] == HIPMC_BIJ_EXP ? ex[k] : th[k];
       }
     }
     double lpl;
@@ -130,15 +137,17 @@ struct Target {
     for (int k = 0; k < K; ++k) {
       const T xv = xs[k];
       if (sp_slots & (1u << k)) {
+        keep_branch();
         // fldj = -softplus(-x); d/dx f = sigmoid(x); d/dx fldj = sigmoid(-x)
         const bool on = code[k] == HIPMC_BIJ_SOFTPLUS;
-        lpl += on ? (double)(-softplus(-xv)) : 0.0;
-        g[k] = on ? g[k] * sigmoid(xv) + sigmoid(-xv) : g[k];
+        lpl += on ? (double)(-(fmax(-xv, T(0)) + sp[k].l)) : 0.0;
+        g[k] = on ? g[k] * sp[k].sig + sp[k].sig_neg : g[k];
       }
       if (ex_slots & (1u << k)) {
+        keep_branch();
         const bool on = code[k] == HIPMC_BIJ_EXP;
         lpl += on ? (double)xv : 0.0;
-        g[k] = on ? g[k] * exp(xv) + T(1) : g[k];
+        g[k] = on ? g[k] * ex[k] + T(1) : g[k];
       }
     }
     return lpl;
@@ -175,12 +184,11 @@ struct Target {
     T S1, S2;
     wave_sum2(s1, s2, S1, S2);
     const T gmu = -mu * inv_s0sq + S1;
-    const T gtau = T(-2) * tau / (s1sq + tau * tau) + S2;
-    if (lane == 0) g[0] = gmu;
-    if (lane == 1) g[0] = gtau;
+    const T gtau = T(-2) * tau * recip(s1sq + tau * tau) + S2;
+    g[0] = lane == 0 ? gmu : (lane == 1 ? gtau : g[0]);
     const T zm = mu * inv_s0;
     const T zt = tau * inv_s1;
-    lpu = log_norm + (double)(T(-0.5) * zm * zm) - (double)log1p(zt * zt);
+    lpu = log_norm + (double)(T(-0.5) * zm * zm) - (double)log1p_pos(zt * zt);
     if (tau < T(0)) lpu = neg_inf();
     return lpl;
   }
@@ -293,10 +301,11 @@ struct Target {
         ssq += valid ? res * res : T(0);
       } else {
         const T w0 = T(1) - yv;
-        const T l0 = w0 == T(0) ? T(0) : -softplus(s) * w0;
-        const T l1 = yv == T(0) ? T(0) : -softplus(-s) * yv;
+        const SoftplusParts<T> sp = softplus_parts(s);
+        const T l0 = w0 == T(0) ? T(0) : -(fmax(s, T(0)) + sp.l) * w0;
+        const T l1 = yv == T(0) ? T(0) : -(fmax(-s, T(0)) + sp.l) * yv;
         lpl += valid ? (double)(l0 + l1) : 0.0;
-        r = valid ? yv - sigmoid(s) : T(0);
+        r = valid ? yv - sp.sig : T(0);
       }
       __builtin_amdgcn_wave_barrier();
       rbuf[lane] = r;

@@ -322,6 +322,8 @@ def _run_fused(plan, num_results, current_state, num_burnin_steps, thin,
     if 'out_leapfrogs' in outs:
       stats['leapfrogs_total'] = int(outs['out_leapfrogs'][:n_total].sum())
       stats['leapfrogs_per_step'] = outs['out_leapfrogs'][:n_total].sum(1)
+      stats['leapfrogs_max_per_step'] = (
+          outs['out_leapfrogs'][:n_total].max(1).values)
     stats['n_steps'] = n_total
     stats['n_chains'] = c
     if events is not None:
