@@ -13,7 +13,18 @@ achieved FP32 rate against the MI355X FP32 peak (HIP events over every
 sampling launch), and the CPU restatement of TFP's batched algorithm timed
 on a bounded sample of transitions on one host core.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+Other workloads (one JSON line each, same fields):
+  --workload logistic_1024x25     plain HMC, logistic regression 1024x25,
+                                  64 chains, L=10, 500 + 1000 steps
+  --workload logistic_65536x256   HMC + dual averaging on the BF16 MFMA
+                                  gradient path, 4096 chains x 256, L=20
+  --workload ragged_glms_512      512 ragged GLMs x 32 chains, NUTS with
+                                  per-problem dual averaging, batched vs one
+                                  posterior per launch
+  --workload serving_poisson200   continuous-batching server, Poisson
+                                  arrivals at 200 req/s
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload W]
 """
 
 import argparse
@@ -51,6 +62,14 @@ def parse():
                   help='override the number of draws (profiling runs)')
   ap.add_argument('--calib-copy', action='store_true',
                   help='run one 256 MiB device copy (counter calibration)')
+  ap.add_argument('--workload', default='hier_normal_j256',
+                  choices=['hier_normal_j256', 'logistic_1024x25',
+                           'logistic_65536x256', 'ragged_glms_512',
+                           'serving_poisson200'])
+  ap.add_argument('--requests', type=int, default=2000,
+                  help='serving: number of requests')
+  ap.add_argument('--plain-sample', type=int, default=16,
+                  help='ragged GLMs: problems timed one per launch')
   return ap.parse_args()
 
 
@@ -144,6 +163,11 @@ def main():
   import torch
   import probability_amd as pa
   from probability_amd import workloads
+  if args.workload != 'hier_normal_j256':
+    import bench_workloads
+    line = bench_workloads.run(args, torch, pa)
+    print(json.dumps(line), flush=True)
+    return
 
   wl = workloads.hier_normal_j256()
   if args.adapt is not None:

@@ -390,10 +390,9 @@ __global__ void __launch_bounds__(256) nuts_kernel(hipmc_run_args a) {
           eS = energy;
         }
         wS = wsum;
-        if (lane == li) {
-          de_buf = de;
-          de_ok = not_div;  // energy_diff_sum counts steps that continue
-        }
+        // energy_diff_sum counts the steps that continue
+        de_buf = lane == li ? de : de_buf;
+        de_ok = lane == li ? not_div : de_ok;
         nd = nd && not_div;
         csub = not_div && no_u;
         if (li == 63 || !csub || i + 1 == nsteps) {

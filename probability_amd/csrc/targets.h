@@ -28,6 +28,15 @@ __host__ __device__ inline int family_of(int kind) {
   }
 }
 
+// v[s] for a wave-uniform slot index s (selects, no indexed registers).
+template <typename T, int K>
+__device__ inline T pick(const T (&v)[K], int s) {
+  T r = v[0];
+#pragma unroll
+  for (int k = 1; k < K; ++k) r = s == k ? v[k] : r;
+  return r;
+}
+
 template <typename T, int K, int FAM>
 struct Target {
   int kind, dim, n_obs;
@@ -42,6 +51,11 @@ struct Target {
   // wave-uniform: bit k set if some lane of slot k uses that bijector
   int code[K];
   uint32_t sp_slots, ex_slots;
+  // when every bijector lane sits in one register slot (the common case:
+  // a few constrained scalars), that slot and this lane's code in it
+  bool one_slot;
+  int bslot, bcode;
+  bool sp0;  // the only bijector is Softplus, all of it in slot 0
   T inv_s0sq, inv_s0, inv_s1, s1sq;  // prior constants (hier)
   T* lds;  // per-wave scratch: GLM beta [K*64] followed by r [64]
 
@@ -72,6 +86,13 @@ struct Target {
       if (__ballot(c == HIPMC_BIJ_SOFTPLUS)) sp_slots |= 1u << k;
       if (__ballot(c == HIPMC_BIJ_EXP)) ex_slots |= 1u << k;
     }
+    const uint32_t any = sp_slots | ex_slots;
+    one_slot = any != 0u && (any & (any - 1u)) == 0u;
+    sp0 = sp_slots == 1u && ex_slots == 0u;
+    bslot = any ? __builtin_ctz(any) : 0;
+    bcode = code[0];
+#pragma unroll
+    for (int k = 1; k < K; ++k) bcode = bslot == k ? code[k] : bcode;
     inv_s0 = T(1) / p0;
     inv_s0sq = inv_s0 * inv_s0;
     inv_s1 = T(1) / p1;
@@ -104,20 +125,41 @@ struct Target {
     SoftplusParts<T> sp[K];
     T ex[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      th[k] = xs[k];
-      if (sp_slots & (1u << k)) {
+    for (int k = 0; k < K; ++k) th[k] = xs[k];
+    // one bijector slot: select it, transform it, put it back (no per-slot
+    // branches); otherwise the per-slot loop
+    SoftplusParts<T> spb{};
+    T exb = T(0);
+    if (sp0) {
+      spb = softplus_parts(xs[0]);
+      th[0] = code[0] == HIPMC_BIJ_SOFTPLUS ? fmax(xs[0], T(0)) + spb.l
+                                            : xs[0];
+    } else if (one_slot) {
+      const T xb = pick(xs, bslot);
+      if (sp_slots) spb = softplus_parts(xb);
+      if (ex_slots) {
         keep_branch();
-        // Softplus bijector forward (bijectors/softplus.py:36-50): below the
-        // cutoff TFP takes log1p(exp(x)), which equals max(x, 0) + l there.
-        sp[k] = softplus_parts(xs[k]);
-        const T f = fmax(xs[k], T(0)) + sp[k].l;
-        th[k] = code[k] == HIPMC_BIJ_SOFTPLUS ? f : th[k];
+        exb = exp(xb);
       }
-      if (ex_slots & (1u << k)) {
-        keep_branch();
-        ex[k] = exp(xs[k]);
-        th[k] = code[k] == HIPMC_BIJ_EXP ? ex[k] : th[k];
+      // Softplus forward (bijectors/softplus.py:36-50): below the cutoff TFP
+      // takes log1p(exp(x)), which equals max(x, 0) + l there.
+      const T f = bcode == HIPMC_BIJ_SOFTPLUS ? fmax(xb, T(0)) + spb.l
+                  : (bcode == HIPMC_BIJ_EXP ? exb : xb);
+#pragma unroll
+      for (int k = 0; k < K; ++k) th[k] = bslot == k ? f : th[k];
+    } else if (sp_slots | ex_slots) {
+      keep_branch();
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        if (sp_slots & (1u << k)) {
+          sp[k] = softplus_parts(xs[k]);
+          const T f = fmax(xs[k], T(0)) + sp[k].l;
+          th[k] = code[k] == HIPMC_BIJ_SOFTPLUS ? f : th[k];
+        }
+        if (ex_slots & (1u << k)) {
+          ex[k] = exp(xs[k]);
+          th[k] = code[k] == HIPMC_BIJ_EXP ? ex[k] : th[k];
+        }
       }
     }
     double lpl;
@@ -133,21 +175,38 @@ struct Target {
         default: lpl = nan_exp(th, g, lpu); break;
       }
     }
+    // fldj: Softplus -softplus(-x) with d/dx f = sigmoid(x),
+    // d/dx fldj = sigmoid(-x); Exp x with d/dx f = exp(x), d/dx fldj = 1
+    if (sp0) {
+      const bool on = code[0] == HIPMC_BIJ_SOFTPLUS;
+      lpl += on ? (double)(-(fmax(-xs[0], T(0)) + spb.l)) : 0.0;
+      g[0] = on ? g[0] * spb.sig + spb.sig_neg : g[0];
+    } else if (one_slot) {
+      const T xb = pick(xs, bslot);
+      const T gb = pick(g, bslot);
+      const bool on_sp = bcode == HIPMC_BIJ_SOFTPLUS;
+      const bool on_ex = bcode == HIPMC_BIJ_EXP;
+      lpl += on_sp ? (double)(-(fmax(-xb, T(0)) + spb.l))
+                   : (on_ex ? (double)xb : 0.0);
+      const T gn = on_sp ? gb * spb.sig + spb.sig_neg
+                         : (on_ex ? gb * exb + T(1) : gb);
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-      const T xv = xs[k];
-      if (sp_slots & (1u << k)) {
-        keep_branch();
-        // fldj = -softplus(-x); d/dx f = sigmoid(x); d/dx fldj = sigmoid(-x)
-        const bool on = code[k] == HIPMC_BIJ_SOFTPLUS;
-        lpl += on ? (double)(-(fmax(-xv, T(0)) + sp[k].l)) : 0.0;
-        g[k] = on ? g[k] * sp[k].sig + sp[k].sig_neg : g[k];
-      }
-      if (ex_slots & (1u << k)) {
-        keep_branch();
-        const bool on = code[k] == HIPMC_BIJ_EXP;
-        lpl += on ? (double)xv : 0.0;
-        g[k] = on ? g[k] * ex[k] + T(1) : g[k];
+      for (int k = 0; k < K; ++k) g[k] = bslot == k ? gn : g[k];
+    } else if (sp_slots | ex_slots) {
+      keep_branch();
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const T xv = xs[k];
+        if (sp_slots & (1u << k)) {
+          const bool on = code[k] == HIPMC_BIJ_SOFTPLUS;
+          lpl += on ? (double)(-(fmax(-xv, T(0)) + sp[k].l)) : 0.0;
+          g[k] = on ? g[k] * sp[k].sig + sp[k].sig_neg : g[k];
+        }
+        if (ex_slots & (1u << k)) {
+          const bool on = code[k] == HIPMC_BIJ_EXP;
+          lpl += on ? (double)xv : 0.0;
+          g[k] = on ? g[k] * ex[k] + T(1) : g[k];
+        }
       }
     }
     return lpl;
@@ -250,6 +309,58 @@ struct Target {
   // s = X beta is computed with one row per lane (X^T reads coalesced);
   // X^T r is accumulated with one state element per lane (X reads
   // coalesced), r passing through LDS 64 rows at a time.
+  // gl[k] += sum_j X[j, k*64 + lane] r[j] over `rows` rows of row-major X
+  // (a lane per coefficient).  Unswitched on the number of live slots; the
+  // loads are unconditional (lanes past dx read column dx - 1 and are
+  // masked) so eight rows of loads stay in flight.
+  template <int KK>
+  __device__ void grad_rows_k(const T* xrow, int dx, int rows, const T* rbuf,
+                              T (&gl)[K]) {
+    const int lane = lane_id();
+    int col[KK];
+    bool on[KK];
+#pragma unroll
+    for (int k = 0; k < KK; ++k) {
+      const int e = k * 64 + lane;
+      on[k] = e < dx;
+      col[k] = on[k] ? e : dx - 1;
+    }
+    int j = 0;
+    for (; j + 8 <= rows; j += 8) {
+      T v[8][KK];
+#pragma unroll
+      for (int u = 0; u < 8; ++u)
+#pragma unroll
+        for (int k = 0; k < KK; ++k) v[u][k] = xrow[(size_t)(j + u) * dx + col[k]];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const T rj = rbuf[j + u];
+#pragma unroll
+        for (int k = 0; k < KK; ++k) gl[k] += (on[k] ? v[u][k] : T(0)) * rj;
+      }
+    }
+    for (; j < rows; ++j) {
+      const T rj = rbuf[j];
+#pragma unroll
+      for (int k = 0; k < KK; ++k)
+        gl[k] += (on[k] ? xrow[(size_t)j * dx + col[k]] : T(0)) * rj;
+    }
+  }
+
+  __device__ void grad_rows(const T* xrow, int dx, int rows, const T* rbuf,
+                            T (&gl)[K]) {
+    const int live = (dx + 63) >> 6;
+    if (K == 1 || live <= 1) {
+      grad_rows_k<1>(xrow, dx, rows, rbuf, gl);
+    } else if (K == 2 || live == 2) {
+      grad_rows_k<(K >= 2 ? 2 : 1)>(xrow, dx, rows, rbuf, gl);
+    } else if (K == 3 || live == 3) {
+      grad_rows_k<(K >= 3 ? 3 : 1)>(xrow, dx, rows, rbuf, gl);
+    } else {
+      grad_rows_k<K>(xrow, dx, rows, rbuf, gl);
+    }
+  }
+
   template <bool LINEAR>
   __device__ double glm(const T (&th)[K], T (&g)[K], double& lpu) {
     const int lane = lane_id();
@@ -280,18 +391,24 @@ struct Target {
     for (int n0 = 0; n0 < n; n0 += 64) {
       const int row = n0 + lane;
       const bool valid = row < n;
-      T s = T(0);
-      if (valid) {
-        const T* col = xt + row;
-        int d = 0;
-        for (; d + 4 <= dx; d += 4) {
-          s += col[(size_t)d * n] * beta[d];
-          s += col[(size_t)(d + 1) * n] * beta[d + 1];
-          s += col[(size_t)(d + 2) * n] * beta[d + 2];
-          s += col[(size_t)(d + 3) * n] * beta[d + 3];
+      // s = x_row . beta: a lane per row, coalesced over X^T; rows past the
+      // end read row n - 1 and are masked, so the loads carry no branch and
+      // eight of them stay in flight
+      const T* col = xt + (valid ? row : n - 1);
+      T s0 = T(0), s1 = T(0);
+      int d = 0;
+      for (; d + 8 <= dx; d += 8) {
+        T v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = col[(size_t)(d + u) * n];
+#pragma unroll
+        for (int u = 0; u < 8; u += 2) {
+          s0 += v[u] * beta[d + u];
+          s1 += v[u + 1] * beta[d + u + 1];
         }
-        for (; d < dx; ++d) s += col[(size_t)d * n] * beta[d];
       }
+      for (; d < dx; ++d) s0 += col[(size_t)d * n] * beta[d];
+      const T s = s0 + s1;
       const T yv = valid ? y[row] : T(0);
       T r;
       if (LINEAR) {
@@ -313,18 +430,7 @@ struct Target {
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
       const int rows = (n - n0) < 64 ? (n - n0) : 64;
-      const T* xrow = x + (size_t)n0 * dx;
-      for (int j = 0; j < rows; ++j) {
-        const T rj = rbuf[j];
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-          const int e = k * 64 + lane;
-          if (k * 64 < dx) {
-            if (e < dx) gl[k] += xrow[e] * rj;
-          }
-        }
-        xrow += dx;
-      }
+      grad_rows(x + (size_t)n0 * dx, dx, rows, rbuf, gl);
     }
     double lp = log_norm;
 #pragma unroll

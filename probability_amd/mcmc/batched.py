@@ -240,6 +240,10 @@ def sample_posteriors(batch, num_results, num_burnin_steps=0,
   if stats is not None:
     stats['leapfrogs_per_step'] = lf[:n_total].sum(1)
     stats['leapfrogs_total'] = int(lf[:n_total].sum())
+    stats['leapfrogs_per_problem'] = torch.zeros(
+        p, dtype=torch.float64, device=dev).index_add_(
+            0, batch.chain_problem.long(), lf[:n_total].sum(0).double()
+        ).cpu().numpy()
     if events is not None:
       torch.cuda.synchronize(dev)
       stats['launches'] = [(e0.elapsed_time(e1), t0, n)

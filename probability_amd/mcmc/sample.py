@@ -273,6 +273,12 @@ def _run_fused(plan, num_results, current_state, num_burnin_steps, thin,
 
   fn = (_lib.lib().hipmc_nuts_run if plan.kind == 'nuts'
         else _lib.lib().hipmc_hmc_run)
+  # Large-batch logistic HMC runs its all-chain gradients as MFMA GEMMs.
+  mfma = (plan.kind == 'hmc' and plan.ttk is None and
+          isinstance(spec, targets_lib.LogisticRegressionBF16) and
+          spec.mfma_ok(c) and step.numel() == 1)
+  if mfma and tlp_hist is not None:
+    return None
   stream = _lib.stream_handle(dev)
   t = 0
   while t < n_total:
@@ -282,29 +288,25 @@ def _run_fused(plan, num_results, current_state, num_burnin_steps, thin,
       n = min(n_total - t, max_iters_per_launch)
     if step_hist is not None:
       step_hist[t:t + n] = step.reshape(step_hist.shape[1:])
-    run = _runner.Run(spec, table, state, tlp, grad, step, base_seed,
-                      n_iters=n, fold_seed=True, iter0=t)
-    a = run.args
-    if plan.kind == 'nuts':
-      plan.base._configure(run)
+    if mfma:
+      if events is not None:
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+      plan.base.mfma_runner(spec, c).run(
+          state, tlp, grad, step, base_seed, plan.base.num_leapfrog_steps,
+          n_iters=n, fold_seed=True, iter0=t, out_lar=lar_all[t],
+          out_acc=(outs['out_is_accepted'][t] if 'out_is_accepted' in outs
+                   else None),
+          trace=trace_state, trace_start=num_burnin_steps,
+          trace_every=1 + thin, n_trace=num_results)
+      if events is not None:
+        e1.record()
+        events.append((e0, e1, t, n))
     else:
-      a.num_leapfrog_steps = int(plan.base.num_leapfrog_steps)
-    a.out_row0 = t
-    a.out_log_accept_ratio = lar_all.data_ptr()
-    for f, buf in outs.items():
-      setattr(a, f, buf.data_ptr())
-    a.trace_state = trace_state.data_ptr()
-    a.trace_start = num_burnin_steps
-    a.trace_every = 1 + thin
-    a.n_trace = num_results
-    if events is not None:
-      e0 = torch.cuda.Event(enable_timing=True)
-      e1 = torch.cuda.Event(enable_timing=True)
-      e0.record()
-    _lib.check(fn(a, stream))
-    if events is not None:
-      e1.record()
-      events.append((e0, e1, t, n))
+      _launch_fused(plan, spec, table, state, tlp, grad, step, base_seed, n,
+                    t, lar_all, outs, trace_state, num_burnin_steps, thin,
+                    num_results, fn, stream, events)
     if tlp_hist is not None and n == 1:
       tlp_hist[t] = tlp
     if plan.da is not None:
@@ -317,7 +319,6 @@ def _run_fused(plan, num_results, current_state, num_burnin_steps, thin,
           float(pkr.step_count_smoothing), float(pkr.decay_rate),
           _lib.ptr(step), _lib.ptr(step64), stream))
     t += n
-
   if stats is not None:
     if 'out_leapfrogs' in outs:
       stats['leapfrogs_total'] = int(outs['out_leapfrogs'][:n_total].sum())
@@ -374,6 +375,35 @@ def _run_fused(plan, num_results, current_state, num_burnin_steps, thin,
   if trace_fn is None:
     return all_states
   return StatesAndTrace(all_states, trace)
+
+
+def _launch_fused(plan, spec, table, state, tlp, grad, step, base_seed, n, t,
+                  lar_all, outs, trace_state, num_burnin_steps, thin,
+                  num_results, fn, stream, events):
+  """One sampler launch of the fused loop over iterations [t, t + n)."""
+  run = _runner.Run(spec, table, state, tlp, grad, step, base_seed,
+                    n_iters=n, fold_seed=True, iter0=t)
+  a = run.args
+  if plan.kind == 'nuts':
+    plan.base._configure(run)
+  else:
+    a.num_leapfrog_steps = int(plan.base.num_leapfrog_steps)
+  a.out_row0 = t
+  a.out_log_accept_ratio = lar_all.data_ptr()
+  for f, buf in outs.items():
+    setattr(a, f, buf.data_ptr())
+  a.trace_state = trace_state.data_ptr()
+  a.trace_start = num_burnin_steps
+  a.trace_every = 1 + thin
+  a.n_trace = num_results
+  if events is not None:
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+  _lib.check(fn(a, stream))
+  if events is not None:
+    e1.record()
+    events.append((e0, e1, t, n))
 
 
 def _final_results(plan, pkr, spec, state, tlp, grad, step, cs, lar_all, outs,
