@@ -39,6 +39,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 FP32_PEAK_TFLOPS = 157.3
+# HBM bytes of one adaptation launch of nuts_kernel on this workload:
+# rocprofv3 FETCH_SIZE (1390 KiB) x 2.00 + WRITE_SIZE (4643 KiB) x 1.00, the
+# factors being what the two counters read for a 256 MiB device copy.
+NUTS_LAUNCH_TRAFFIC_BYTES = (1390 * 2 + 4643) * 1024
 # Algorithmic FP32 work of one chain-leapfrog on the hierarchical normal:
 # 13 FLOP per group for log-prob + gradient (theta, residual, two scalings,
 # the two quadratic terms, grad_z, the two reductions), 8 FLOP per state
@@ -264,7 +268,10 @@ def main():
       'roofline': {
           'bound': 'mfma', 'achieved': achieved, 'peak': FP32_PEAK_TFLOPS,
           'unit': 'TFLOP/s', 'frac': achieved / FP32_PEAK_TFLOPS,
-          'traffic': None,
+          'traffic': NUTS_LAUNCH_TRAFFIC_BYTES,
+          'traffic_what': 'HBM bytes per adaptation launch (rocprofv3 '
+                          'FETCH_SIZE/WRITE_SIZE, copy-calibrated; '
+                          'profiles/nuts_kernel_hier_normal_j256_traffic.md)',
           'kernel': 'nuts_kernel<float,5,hier>',
           'avg_launch_ms': float(kern_ms.mean()),
           'launches': len(kern_ms),

@@ -172,7 +172,9 @@ def logistic_65536x256(args, torch, pa):
 
   steps = max(1, min(args.steps, 1))
   _say('mfma: runs')
-  res, ms = _timed(torch, run, steps, min(args.warmup, 1))
+  stats = {'time_kernels': True}
+  res, ms = _timed(torch, lambda k: run(k, stats=stats), steps,
+                   min(args.warmup, 1))
   _say(f'mfma: {ms:.0f} ms; diagnostics')
   draws = res.all_states
   acc = float(res.trace.float().mean())
@@ -189,8 +191,6 @@ def logistic_65536x256(args, torch, pa):
   mcse = sd / torch.sqrt(ess.double() / 2)
   z_halves = float(((m1 - m2).abs() / (math.sqrt(2) * mcse)).max())
   # kernel time of the gradient GEMMs: HIP events per launch
-  stats = {'time_kernels': True}
-  run(0, stats=stats)
   launches = stats['launches']
   sample_ms = sum(l[0] for l in launches if l[2] > 1)
   sample_iters = sum(l[2] for l in launches if l[2] > 1)
@@ -239,20 +239,31 @@ def _ragged_specs(torch, pa, n_problems):
 
 def ragged_glms_512(args, torch, pa):
   from probability_amd.mcmc import batched
-  p, chains, na, nr = 512, 32, 500, 1000
+  p, chains = 512, 32
+  na = 500 if args.adapt is None else args.adapt
+  nr = 1000 if args.draws is None else args.draws
   specs, bij = _ragged_specs(torch, pa, p)
   slot = 192
   batch = batched.PosteriorBatch(specs, chains, bijectors=bij, slot_dim=slot)
+  # the sampling phase runs 100 transitions per launch so that progress is
+  # visible between launches
   kw = dict(num_results=nr, num_burnin_steps=na, num_adaptation_steps=na,
-            step_size=0.05, target_accept_prob=0.8, max_tree_depth=10)
+            step_size=0.05, target_accept_prob=0.8, max_tree_depth=10,
+            max_iters_per_launch=100)
+
+  def report(t, n):
+    if t % 25 == 0 or t == n:
+      torch.cuda.synchronize()
+      _say(f'ragged: {t}/{n} transitions')
 
   def run(k, stats=None):
     return batched.sample_posteriors(batch, seed=args.seed + k, stats=stats,
-                                     **kw)
+                                     progress=report, **kw)
 
   _say('ragged: batched runs')
-  res, ms = _timed(torch, run, max(1, min(args.steps, 1)),
-                   min(args.warmup, 1))
+  stats = {'time_kernels': True}
+  res, ms = _timed(torch, lambda k: run(k, stats=stats),
+                   max(1, min(args.steps, 1)), min(args.warmup, 1))
   _say(f'ragged: batched {ms:.0f} ms; diagnostics')
   min_ess, rhat_max = [], []
   for dr in res.draws:
@@ -261,8 +272,6 @@ def ragged_glms_512(args, torch, pa):
     rhat_max.append(float(r.max()))
   min_ess = np.array(min_ess)
   rhat_max = np.array(rhat_max)
-  stats = {'time_kernels': True}
-  run(0, stats=stats)
   lf_total = stats['leapfrogs_total']
   nd = sum(s.n_obs * s.dim for s in specs)
   # one posterior per launch on a stratified sample of problems (the same
@@ -274,7 +283,7 @@ def ragged_glms_512(args, torch, pa):
     one = batched.PosteriorBatch([specs[i]], chains, bijectors=[bij[i]],
                                  keys=[i], slot_dim=slot)
     _, pm = _timed(torch, lambda k: batched.sample_posteriors(
-        one, seed=args.seed + k, **kw), 1, 0)
+        one, seed=args.seed + k, progress=report, **kw), 1, 0)
     plain_ms.append(pm)
   plain_total_ms = float(np.mean(plain_ms)) * p
   agg = float(min_ess.sum()) / (ms / 1e3)
@@ -284,7 +293,7 @@ def ragged_glms_512(args, torch, pa):
                {'workload': 'ragged_glms_512: 256 logistic + 256 linear '
                             '(Exp on sigma), N~LogU[128,8192], D~U{4..128}, '
                             '32 chains, NUTS(depth 10) + per-problem DA '
-                            '(target 0.8), 500 adaptation + 1000 draws',
+                            f'(target 0.8), {na} adaptation + {nr} draws',
                 'problems': p, 'chains_per_problem': chains,
                 'sum_n_times_d': nd})
   line.update({
@@ -368,7 +377,7 @@ def serving_poisson200(args, torch, pa):
       ess_min.append(float(e.min()))
       rhat_max.append(float(rh.max()))
       req.draws = None
-    if time.perf_counter() - last_print > 30:
+    if time.perf_counter() - last_print > 20:
       last_print = time.perf_counter()
       print(f'serving: {len(lat)}/{n_req} done, {len(srv.active)} active, '
             f'{len(srv.queue)} queued, t={last_print - t0:.0f}s', flush=True)

@@ -28,6 +28,10 @@ __host__ __device__ inline int family_of(int kind) {
   }
 }
 
+// Loads a GLM evaluation keeps in flight per wave (one wave per chain is
+// latency bound on X, so memory-level parallelism sets its speed).
+constexpr int kGlmBatch = 16;
+
 // v[s] for a wave-uniform slot index s (selects, no indexed registers).
 template <typename T, int K>
 __device__ inline T pick(const T (&v)[K], int s) {
@@ -312,7 +316,7 @@ struct Target {
   // gl[k] += sum_j X[j, k*64 + lane] r[j] over `rows` rows of row-major X
   // (a lane per coefficient).  Unswitched on the number of live slots; the
   // loads are unconditional (lanes past dx read column dx - 1 and are
-  // masked) so eight rows of loads stay in flight.
+  // masked) so kGlmBatch (then 4, then 1) rows of loads stay in flight.
   template <int KK>
   __device__ void grad_rows_k(const T* xrow, int dx, int rows, const T* rbuf,
                               T (&gl)[K]) {
@@ -326,14 +330,27 @@ struct Target {
       col[k] = on[k] ? e : dx - 1;
     }
     int j = 0;
-    for (; j + 8 <= rows; j += 8) {
-      T v[8][KK];
+    for (; j + kGlmBatch <= rows; j += kGlmBatch) {
+      T v[kGlmBatch][KK];
 #pragma unroll
-      for (int u = 0; u < 8; ++u)
+      for (int u = 0; u < kGlmBatch; ++u)
 #pragma unroll
         for (int k = 0; k < KK; ++k) v[u][k] = xrow[(size_t)(j + u) * dx + col[k]];
 #pragma unroll
-      for (int u = 0; u < 8; ++u) {
+      for (int u = 0; u < kGlmBatch; ++u) {
+        const T rj = rbuf[j + u];
+#pragma unroll
+        for (int k = 0; k < KK; ++k) gl[k] += (on[k] ? v[u][k] : T(0)) * rj;
+      }
+    }
+    for (; j + 4 <= rows; j += 4) {
+      T v[4][KK];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int k = 0; k < KK; ++k) v[u][k] = xrow[(size_t)(j + u) * dx + col[k]];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
         const T rj = rbuf[j + u];
 #pragma unroll
         for (int k = 0; k < KK; ++k) gl[k] += (on[k] ? v[u][k] : T(0)) * rj;
@@ -392,20 +409,27 @@ struct Target {
       const int row = n0 + lane;
       const bool valid = row < n;
       // s = x_row . beta: a lane per row, coalesced over X^T; rows past the
-      // end read row n - 1 and are masked, so the loads carry no branch and
-      // eight of them stay in flight
+      // end read row n - 1 and are masked, so the loads carry no branch.
+      // Batches of kGlmBatch, then 4, then 1 keep loads in flight.
       const T* col = xt + (valid ? row : n - 1);
       T s0 = T(0), s1 = T(0);
       int d = 0;
-      for (; d + 8 <= dx; d += 8) {
-        T v[8];
+      for (; d + kGlmBatch <= dx; d += kGlmBatch) {
+        T v[kGlmBatch];
 #pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = col[(size_t)(d + u) * n];
+        for (int u = 0; u < kGlmBatch; ++u) v[u] = col[(size_t)(d + u) * n];
 #pragma unroll
-        for (int u = 0; u < 8; u += 2) {
+        for (int u = 0; u < kGlmBatch; u += 2) {
           s0 += v[u] * beta[d + u];
           s1 += v[u + 1] * beta[d + u + 1];
         }
+      }
+      for (; d + 4 <= dx; d += 4) {
+        T v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = col[(size_t)(d + u) * n];
+        s0 += v[0] * beta[d] + v[2] * beta[d + 2];
+        s1 += v[1] * beta[d + 1] + v[3] * beta[d + 3];
       }
       for (; d < dx; ++d) s0 += col[(size_t)d * n] * beta[d];
       const T s = s0 + s1;

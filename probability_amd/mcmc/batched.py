@@ -139,13 +139,14 @@ def sample_posteriors(batch, num_results, num_burnin_steps=0,
                       target_accept_prob=0.75, max_tree_depth=10,
                       max_energy_diff=1000.0, kernel='nuts',
                       num_leapfrog_steps=10, seed=None, trace=True,
-                      stats=None, max_iters_per_launch=4096):
+                      stats=None, max_iters_per_launch=4096, progress=None):
   """Runs every problem of `batch` for num_burnin_steps + num_results
   transitions (dual averaging over the first num_adaptation_steps) and
   returns per-problem draws in the constrained space.
 
   init_state: [C, dim_stride] constrained initial states (None: unconstrained
-  zeros).  step_size: scalar or one value per problem.
+  zeros).  step_size: scalar or one value per problem.  progress, if given,
+  is called as progress(transitions_enqueued, total) after each launch.
   """
   if kernel not in ('nuts', 'hmc'):
     raise ValueError("kernel must be 'nuts' or 'hmc'")
@@ -237,6 +238,8 @@ def sample_posteriors(batch, num_results, num_burnin_steps=0,
     if num_adaptation_steps > 0:
       da(t, n)
     t += n
+    if progress is not None:
+      progress(t, n_total)
   if stats is not None:
     stats['leapfrogs_per_step'] = lf[:n_total].sum(1)
     stats['leapfrogs_total'] = int(lf[:n_total].sum())
