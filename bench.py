@@ -74,6 +74,9 @@ def parse():
                   help='serving: number of requests')
   ap.add_argument('--plain-sample', type=int, default=16,
                   help='ragged GLMs: problems timed one per launch')
+  ap.add_argument('--glm-waves', type=int, default=None,
+                  help='ragged GLMs: waves sharing each chain (default: by '
+                       'problem size)')
   return ap.parse_args()
 
 

@@ -244,7 +244,8 @@ def ragged_glms_512(args, torch, pa):
   nr = 1000 if args.draws is None else args.draws
   specs, bij = _ragged_specs(torch, pa, p)
   slot = 192
-  batch = batched.PosteriorBatch(specs, chains, bijectors=bij, slot_dim=slot)
+  batch = batched.PosteriorBatch(specs, chains, bijectors=bij, slot_dim=slot,
+                                 waves_per_chain=args.glm_waves)
   # the sampling phase runs 100 transitions per launch so that progress is
   # visible between launches
   kw = dict(num_results=nr, num_burnin_steps=na, num_adaptation_steps=na,
@@ -281,7 +282,8 @@ def ragged_glms_512(args, torch, pa):
   for i in idx:
     _say(f'ragged: problem {i} alone')
     one = batched.PosteriorBatch([specs[i]], chains, bijectors=[bij[i]],
-                                 keys=[i], slot_dim=slot)
+                                 keys=[i], slot_dim=slot,
+                                 waves_per_chain=batch.waves_per_chain)
     _, pm = _timed(torch, lambda k: batched.sample_posteriors(
         one, seed=args.seed + k, progress=report, **kw), 1, 0)
     plain_ms.append(pm)
@@ -295,6 +297,7 @@ def ragged_glms_512(args, torch, pa):
                             '32 chains, NUTS(depth 10) + per-problem DA '
                             f'(target 0.8), {na} adaptation + {nr} draws',
                 'problems': p, 'chains_per_problem': chains,
+                'waves_per_chain': batch.waves_per_chain,
                 'sum_n_times_d': nd})
   line.update({
       'min_ess_per_problem_min_median': [float(min_ess.min()),
@@ -313,6 +316,7 @@ def ragged_glms_512(args, torch, pa):
 
 def _ragged_roofline(stats, specs, chains, lf_total):
   kern_ms = sum(l[0] for l in stats['launches'])
+  adapt_ms = sum(l[0] for l in stats['launches'] if l[2] == 1)
   # algorithmic FLOP: 4 N_k D_k per chain-leapfrog, weighted by each
   # problem's share of the leapfrogs (per-problem totals)
   lf = stats['leapfrogs_per_problem']
@@ -322,7 +326,8 @@ def _ragged_roofline(stats, specs, chains, lf_total):
   return {'bound': 'mfma', 'achieved': achieved, 'peak': FP32_PEAK_TFLOPS,
           'unit': 'TFLOP/s', 'frac': achieved / FP32_PEAK_TFLOPS,
           'traffic': None, 'kernel': 'nuts_kernel<float,3,glm>',
-          'kernel_ms': kern_ms, 'chain_leapfrogs': lf_total}
+          'kernel_ms': kern_ms, 'adaptation_kernel_ms': adapt_ms,
+          'chain_leapfrogs': lf_total}
 
 
 # ---------------------------------------------------------- C4 serving

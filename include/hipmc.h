@@ -47,7 +47,7 @@
 extern "C" {
 #endif
 
-#define HIPMC_ABI_VERSION 1
+#define HIPMC_ABI_VERSION 2
 
 /* error codes */
 #define HIPMC_OK 0
@@ -78,7 +78,8 @@ extern "C" {
  * One posterior ("problem").  Lives in DEVICE memory as an array; the
  * layout is fixed (offsets asserted in the library).  Data pointers are in
  * the kernel dtype.
- *   LOGISTIC / LINEAR: x = X [n_obs, dim_x] row-major, xt = X^T [dim_x, n_obs],
+ *   LOGISTIC / LINEAR: x = X [n_obs, dim_x] row-major (row pitch x_pitch),
+ *                      xt = X^T [dim_x, n_obs] (row pitch xt_pitch),
  *                      y [n_obs]; param[0] = prior scale of beta,
  *                      param[1] = HalfNormal scale of sigma (LINEAR).
  *   HIER_NORMAL:       y [J], aux = sigma [J], n_obs = J;
@@ -104,6 +105,11 @@ typedef struct hipmc_problem {
   int32_t bij_len[HIPMC_MAX_BIJ_RANGES];
   int32_t bij_code[HIPMC_MAX_BIJ_RANGES];
   uint32_t key; /* problem word of the Philox counter (stable id) */
+  /* LOGISTIC / LINEAR: row pitches (elements) of x (>= dim_x) and xt
+   * (>= n_obs), multiples of 4 with 16-byte aligned x / xt: rows are read
+   * 16 bytes per lane (padding is read and ignored).  Other kinds: unused. */
+  int32_t x_pitch;
+  int32_t xt_pitch;
 } hipmc_problem;
 
 /*
@@ -165,7 +171,11 @@ typedef struct hipmc_run_args {
   /* problems and chain maps (device) */
   const hipmc_problem* problems;
   int32_t n_problems;
-  int32_t pad1_;
+  /* LOGISTIC / LINEAR launches of hipmc_nuts_run / hipmc_hmc_run: waves that
+   * share one chain's gradient (each takes every waves_per_chain-th block of
+   * rows; partial sums meet in LDS).  0 or 1: one wave per chain.  At most
+   * 4; other target kinds ignore it. */
+  int32_t waves_per_chain;
   const int32_t* chain_problem;
   const int32_t* chain_local;
   /* step size (device, dtype): step_size[chain_step[c]] (chain_step NULL: 0) */

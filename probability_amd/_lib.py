@@ -47,6 +47,7 @@ class Problem(ctypes.Structure):
       ('bij_len', _i32 * MAX_BIJ_RANGES),
       ('bij_code', _i32 * MAX_BIJ_RANGES),
       ('key', ctypes.c_uint32),
+      ('x_pitch', _i32), ('xt_pitch', _i32),
   ]
 
 
@@ -66,7 +67,7 @@ class RunArgs(ctypes.Structure):
       ('num_leapfrog_steps', _i32), ('max_tree_depth', _i32),
       ('unrolled_leapfrog_steps', _i32), ('target_kind', _i32),
       ('max_energy_diff', _f64),
-      ('problems', _vp), ('n_problems', _i32), ('pad1_', _i32),
+      ('problems', _vp), ('n_problems', _i32), ('waves_per_chain', _i32),
       ('chain_problem', _vp), ('chain_local', _vp),
       ('step_size', _vp), ('chain_step', _vp),
       ('state', _vp), ('tlp', _vp), ('grad', _vp),
@@ -165,6 +166,18 @@ def ptr(t):
 
 def stream_handle(device=None):
   return torch.cuda.current_stream(device).cuda_stream
+
+
+def glm_waves_per_chain(max_rows_times_cols):
+  """Waves that share one GLM chain's gradient for a launch whose largest
+  design matrix has `max_rows_times_cols` entries: one wave streams X at
+  about one 16-byte load per lane in flight, so big problems split their
+  rows over up to four waves."""
+  if max_rows_times_cols >= 16384:
+    return 4
+  if max_rows_times_cols >= 4096:
+    return 2
+  return 1
 
 
 def dtype_code(dtype):

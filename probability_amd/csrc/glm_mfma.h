@@ -73,18 +73,38 @@ __global__ void __launch_bounds__(256, 2)
   __bf16* rl_w = rs + kMChains * kRPitch + (16 * w) * kRPitch;
   for (int n0 = r0; n0 < r1; n0 += kMRows) {
     __syncthreads();
-    // stage X[n0:n0+32, :] and X^T[:, n0:n0+32] (16-byte chunks)
-    for (int i = tid; i < kMRows * (kMD / 8); i += 256) {
+    // stage X[n0:n0+32, :] and X^T[:, n0:n0+32] (16-byte chunks): all eight
+    // loads of a thread are issued before the first LDS store, so a tile
+    // costs one global round trip
+    constexpr int kChunksX = kMRows * (kMD / 8) / 256;
+    constexpr int kChunksT = kMD * (kMRows / 8) / 256;
+    uint4 px[kChunksX], pt[kChunksT];
+#pragma unroll
+    for (int u = 0; u < kChunksX; ++u) {
+      const int i = tid + 256 * u;
       const int row = i / (kMD / 8), ch = i % (kMD / 8);
-      *(uint4*)&xs[row * kXsPitch + ch * 8] =
-          *(const uint4*)(X + (size_t)(n0 + row) * kMD + ch * 8);
+      px[u] = *(const uint4*)(X + (size_t)(n0 + row) * kMD + ch * 8);
     }
-    for (int i = tid; i < kMD * (kMRows / 8); i += 256) {
+#pragma unroll
+    for (int u = 0; u < kChunksT; ++u) {
+      const int i = tid + 256 * u;
       const int d = i / (kMRows / 8), ch = i % (kMRows / 8);
-      *(uint4*)&xts[d * kXtPitch + ch * 8] =
-          *(const uint4*)(Xt + (size_t)d * n_obs + n0 + ch * 8);
+      pt[u] = *(const uint4*)(Xt + (size_t)d * n_obs + n0 + ch * 8);
     }
-    if (tid < kMRows) ys[tid] = y[n0 + tid];
+    const float py = tid < kMRows ? y[n0 + tid] : 0.f;
+#pragma unroll
+    for (int u = 0; u < kChunksX; ++u) {
+      const int i = tid + 256 * u;
+      const int row = i / (kMD / 8), ch = i % (kMD / 8);
+      *(uint4*)&xs[row * kXsPitch + ch * 8] = px[u];
+    }
+#pragma unroll
+    for (int u = 0; u < kChunksT; ++u) {
+      const int i = tid + 256 * u;
+      const int d = i / (kMRows / 8), ch = i % (kMRows / 8);
+      *(uint4*)&xts[d * kXtPitch + ch * 8] = pt[u];
+    }
+    if (tid < kMRows) ys[tid] = py;
     __syncthreads();
 
     // S tile [32 rows x 16 chains] of this wave: two 16x16 blocks.
@@ -131,11 +151,18 @@ __global__ void __launch_bounds__(256, 2)
     // G[16 chains x 256] += R^T[16 x 32 rows] X[32 rows x 256]
     const bf16x8 ah = *(const bf16x8*)&rh_w[m * kRPitch + 8 * g];
     const bf16x8 al = *(const bf16x8*)&rl_w[m * kRPitch + 8 * g];
+    // four B fragments are read from LDS ahead of their eight MFMAs
 #pragma unroll
-    for (int nt = 0; nt < kMD / 16; ++nt) {
-      const bf16x8 b = *(const bf16x8*)&xts[(16 * nt + m) * kXtPitch + 8 * g];
-      acc[nt] = mfma16(ah, b, acc[nt]);
-      acc[nt] = mfma16(al, b, acc[nt]);
+    for (int nt0 = 0; nt0 < kMD / 16; nt0 += 4) {
+      bf16x8 b[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+        b[u] = *(const bf16x8*)&xts[(16 * (nt0 + u) + m) * kXtPitch + 8 * g];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        acc[nt0 + u] = mfma16(ah, b[u], acc[nt0 + u]);
+        acc[nt0 + u] = mfma16(al, b[u], acc[nt0 + u]);
+      }
     }
   }
   // lane holds G[chain c0 + 4g + i][16*nt + m]

@@ -17,7 +17,8 @@
 
 using namespace hipmc;
 
-static_assert(sizeof(hipmc_problem) == 152, "hipmc_problem layout");
+static_assert(sizeof(hipmc_problem) == 160, "hipmc_problem layout");
+static_assert(offsetof(hipmc_problem, x_pitch) == 152, "hipmc_problem layout");
 static_assert(sizeof(hipmc_stream) == 48, "hipmc_stream layout");
 static_assert(offsetof(hipmc_problem, param) == 56, "hipmc_problem layout");
 static_assert(offsetof(hipmc_problem, n_bij) == 96, "hipmc_problem layout");
@@ -86,12 +87,26 @@ int check_run(const hipmc_run_args* a, bool nuts) {
 
 template <typename T, int K, int FAM>
 int launch_nuts(const hipmc_run_args& a, hipStream_t s) {
-  const size_t per_wave = nuts_lds_elems<T, K>(a.max_tree_depth) * sizeof(T);
+  if (a.n_chains == 0) return HIPMC_OK;
+  const int W = waves_per_chain<FAM>(a);
+  if (W > 1) {
+    // one chain per block: one U-turn memory, W target scratches and the
+    // two halves of the partial-sum buffer
+    const size_t bytes =
+        ((size_t)2 * a.max_tree_depth * K * 64 +
+         (size_t)W * target_scratch_elems<T, K, FAM>()) * sizeof(T) +
+        2 * glm_red_bytes<T, K>(W);
+    if (bytes > (size_t)kMaxLdsPerBlock)
+      return fail(HIPMC_E_UNSUPPORTED, "shared-chain LDS does not fit");
+    hipLaunchKernelGGL((nuts_kernel<T, K, FAM>), dim3(a.n_chains),
+                       dim3(64 * W), bytes, s, a);
+    return hip_check(hipGetLastError(), "nuts_kernel launch");
+  }
+  const size_t per_wave = nuts_lds_elems<T, K, FAM>(a.max_tree_depth) * sizeof(T);
   if (per_wave > (size_t)kMaxLdsPerBlock)
     return fail(HIPMC_E_UNSUPPORTED, "U-turn memory does not fit in LDS");
   const int w = waves_for_lds(per_wave);
   const int blocks = (a.n_chains + w - 1) / w;
-  if (blocks == 0) return HIPMC_OK;
   hipLaunchKernelGGL((nuts_kernel<T, K, FAM>), dim3(blocks), dim3(64 * w),
                      per_wave * w, s, a);
   return hip_check(hipGetLastError(), "nuts_kernel launch");
@@ -99,10 +114,17 @@ int launch_nuts(const hipmc_run_args& a, hipStream_t s) {
 
 template <typename T, int K, int FAM, bool CAL>
 int launch_hmc(const hipmc_run_args& a, hipStream_t s) {
-  const size_t per_wave = hmc_lds_elems<T, K>() * sizeof(T);
+  if (a.n_chains == 0) return HIPMC_OK;
+  const size_t per_wave = hmc_lds_elems<T, K, FAM>() * sizeof(T);
+  const int W = waves_per_chain<FAM>(a);
+  if (W > 1) {
+    const size_t bytes = per_wave * W + 2 * glm_red_bytes<T, K>(W);
+    hipLaunchKernelGGL((hmc_kernel<T, K, FAM, CAL>), dim3(a.n_chains),
+                       dim3(64 * W), bytes, s, a);
+    return hip_check(hipGetLastError(), "hmc_kernel launch");
+  }
   const int w = 4;
   const int blocks = (a.n_chains + w - 1) / w;
-  if (blocks == 0) return HIPMC_OK;
   hipLaunchKernelGGL((hmc_kernel<T, K, FAM, CAL>), dim3(blocks), dim3(64 * w),
                      per_wave * w, s, a);
   return hip_check(hipGetLastError(), "hmc_kernel launch");
@@ -162,7 +184,7 @@ template <typename T, int K, int FAM>
 struct EvalL {
   static int run(const hipmc_problem* p, const int32_t* cp, int n, int stride,
                  const void* x, void* tlp, void* grad, hipStream_t s) {
-    const size_t per_wave = hmc_lds_elems<T, K>() * sizeof(T);
+    const size_t per_wave = hmc_lds_elems<T, K, FAM>() * sizeof(T);
     const int w = 4;
     const int blocks = (n + w - 1) / w;
     if (blocks == 0) return HIPMC_OK;

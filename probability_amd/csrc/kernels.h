@@ -213,32 +213,54 @@ __device__ inline void put(void* base, size_t i, T v) {
 }
 
 
-// ------------------------------------------------------------------ NUTS
-// LDS per wave (elements of T): U-turn memory momentum [max_depth][K*64],
-// cumulative momentum [max_depth][K*64], then target scratch [K*64 + 64].
-template <typename T, int K>
-__host__ __device__ constexpr size_t nuts_lds_elems(int max_depth) {
-  return (size_t)2 * max_depth * K * 64 + K * 64 + 64;
+// Waves sharing one chain: only GLM launches split a chain (at most 4).
+template <int FAM>
+__host__ __device__ inline int waves_per_chain(const hipmc_run_args& a) {
+  if (FAM != kFamilyGlm || a.waves_per_chain <= 1) return 1;
+  return a.waves_per_chain < 4 ? a.waves_per_chain : 4;
 }
 
+// ------------------------------------------------------------------ NUTS
+// LDS per wave (elements of T): U-turn memory momentum [max_depth][K*64],
+// cumulative momentum [max_depth][K*64], then the target's scratch.
 template <typename T, int K, int FAM>
-__global__ void __launch_bounds__(256) nuts_kernel(hipmc_run_args a) {
+__host__ __device__ constexpr size_t nuts_lds_elems(int max_depth) {
+  return (size_t)2 * max_depth * K * 64 + target_scratch_elems<T, K, FAM>();
+}
+
+// GLM chains are bound by loads in flight, so their kernel is built for two
+// waves per SIMD (the register budget that allows); the other families keep
+// one wave per SIMD and every register for the leaf's latency.
+template <typename T, int K, int FAM>
+__global__ void __launch_bounds__(256)
+    __attribute__((amdgpu_waves_per_eu(FAM == kFamilyGlm ? 2 : 1)))
+    nuts_kernel(hipmc_run_args a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   const int lane = lane_id();
   const int wave = threadIdx.x >> 6;
   const int waves = blockDim.x >> 6;
-  const int c = xcd_block(blockIdx.x, gridDim.x) * waves + wave;
-  if (c >= a.n_chains) return;  // whole wave leaves together
   const int max_depth = a.max_tree_depth;
-  T* wl = (T*)smem_raw + (size_t)wave * nuts_lds_elems<T, K>(max_depth);
-  T* memP = wl;
+  // GLM chains may be shared by the W waves of a block (each evaluates a
+  // share of the rows, the partial sums meet in LDS, every wave runs the
+  // same control flow on the same values); otherwise a wave per chain.
+  const int W = waves_per_chain<FAM>(a);
+  const int c = W > 1 ? xcd_block(blockIdx.x, gridDim.x)
+                      : xcd_block(blockIdx.x, gridDim.x) * waves + wave;
+  const int wic = W > 1 ? wave : 0;
+  if (c >= a.n_chains) return;  // whole wave (block, if shared) leaves
+  const size_t umem = (size_t)2 * max_depth * K * 64;
+  const size_t scr = target_scratch_elems<T, K, FAM>();
+  T* wl = W > 1 ? (T*)smem_raw
+                : (T*)smem_raw + (size_t)wave * nuts_lds_elems<T, K, FAM>(max_depth);
+  T* memP = wl;  // one copy per chain; the waves of a chain write equal values
   T* memC = wl + (size_t)max_depth * K * 64;
-  T* scratch = wl + (size_t)2 * max_depth * K * 64;
+  T* scratch = wl + umem + (size_t)wic * scr;
 
   const ChainId id = chain_of(a, c);
   const hipmc_problem& pb = a.problems[id.prob];
   Target<T, K, FAM> tg;
   tg.load(pb, scratch);
+  if (W > 1) tg.set_split(wic, W, (unsigned char*)(wl + umem + (size_t)W * scr));
   const int dim = pb.dim;
   const int crow = a.chain_row ? a.chain_row[c] : c;
   const size_t row = (size_t)crow * a.dim_stride;
@@ -433,7 +455,7 @@ __global__ void __launch_bounds__(256) nuts_kernel(hipmc_run_args a) {
     // the log-prob crosses iterations in the state dtype, exactly as it does
     // between separate one_step launches
     t0 = (double)(T)tC;
-    if (lane == 0) {
+    if (lane == 0 && wic == 0) {
       const size_t o = (size_t)(a.out_row0 + it) * a.n_chains + c;
       put<T>(a.out_log_accept_ratio, o, (T)log(eds / (double)lf));
       if (a.out_leapfrogs) a.out_leapfrogs[o] = lf * nunroll;
@@ -443,17 +465,18 @@ __global__ void __launch_bounds__(256) nuts_kernel(hipmc_run_args a) {
       put<T>(a.out_energy, o, (T)eC);
       if (a.out_tree_depth) a.out_tree_depth[o] = depth;
     }
-    trace_store(stream, a, id, dim, it, q0);
+    if (wic == 0) trace_store(stream, a, id, dim, it, q0);
   }
+  if (wic != 0) return;
   store_row((T*)a.state + row, q0, dim);
   store_row((T*)a.grad + row, g0, dim);
   if (lane == 0) ((T*)a.tlp)[crow] = (T)t0;
 }
 
 // ------------------------------------------------------------------ HMC
-template <typename T, int K>
+template <typename T, int K, int FAM>
 __host__ __device__ constexpr size_t hmc_lds_elems() {
-  return (size_t)K * 64 + 64;
+  return target_scratch_elems<T, K, FAM>();
 }
 
 // CALIBRATED=true: full MH(HMC) step; false: proposal only (n_iters == 1).
@@ -463,13 +486,20 @@ __global__ void __launch_bounds__(256) hmc_kernel(hipmc_run_args a) {
   const int lane = lane_id();
   const int wave = threadIdx.x >> 6;
   const int waves = blockDim.x >> 6;
-  const int c = xcd_block(blockIdx.x, gridDim.x) * waves + wave;
+  // GLM chains may be shared by the W waves of a block (as in nuts_kernel)
+  const int W = waves_per_chain<FAM>(a);
+  const int c = W > 1 ? xcd_block(blockIdx.x, gridDim.x)
+                      : xcd_block(blockIdx.x, gridDim.x) * waves + wave;
+  const int wic = W > 1 ? wave : 0;
+  const bool writer = wic == 0;  // the wave that stores a shared chain's outputs
   if (c >= a.n_chains) return;
-  T* scratch = (T*)smem_raw + (size_t)wave * hmc_lds_elems<T, K>();
+  const size_t scr = hmc_lds_elems<T, K, FAM>();
+  T* scratch = (T*)smem_raw + (size_t)wave * scr;
   const ChainId id = chain_of(a, c);
   const hipmc_problem& pb = a.problems[id.prob];
   Target<T, K, FAM> tg;
   tg.load(pb, scratch);
+  if (W > 1) tg.set_split(wic, W, (unsigned char*)((T*)smem_raw + (size_t)W * scr));
   const int dim = pb.dim;
   const int crow = a.chain_row ? a.chain_row[c] : c;
   const size_t row = (size_t)crow * a.dim_stride;
@@ -511,7 +541,7 @@ __global__ void __launch_bounds__(256) hmc_kernel(hipmc_run_args a) {
                            acc.k0, acc.k1);
       const double u = (double)uniform_from<T>(uw.x, uw.y);
       const bool accept = log(u) < lar;
-      if (it == stream.n_iters - 1) {
+      if (it == stream.n_iters - 1 && writer) {
         if (a.out_proposed_state) store_row((T*)a.out_proposed_state + row, q, dim);
         if (a.out_initial_momentum) store_row((T*)a.out_initial_momentum + row, p_init, dim);
         if (a.out_final_momentum) store_row((T*)a.out_final_momentum + row, p, dim);
@@ -522,7 +552,7 @@ __global__ void __launch_bounds__(256) hmc_kernel(hipmc_run_args a) {
         copy(g0, g);
         t0 = (double)(T)t;
       }
-      if (lane == 0) {
+      if (lane == 0 && writer) {
         put<T>(a.out_log_accept_ratio, o, (T)lar);
         if (a.out_is_accepted) a.out_is_accepted[o] = accept;
         put<T>(a.out_energy, o, (T)corr);
@@ -531,12 +561,13 @@ __global__ void __launch_bounds__(256) hmc_kernel(hipmc_run_args a) {
       copy(q0, q);
       copy(g0, g);
       t0 = (double)(T)t;
-      if (a.out_initial_momentum) store_row((T*)a.out_initial_momentum + row, p_init, dim);
-      if (a.out_final_momentum) store_row((T*)a.out_final_momentum + row, p, dim);
-      if (lane == 0) put<T>(a.out_energy, o, (T)corr);
+      if (writer && a.out_initial_momentum) store_row((T*)a.out_initial_momentum + row, p_init, dim);
+      if (writer && a.out_final_momentum) store_row((T*)a.out_final_momentum + row, p, dim);
+      if (lane == 0 && writer) put<T>(a.out_energy, o, (T)corr);
     }
-    trace_store(stream, a, id, dim, it, q0);
+    if (writer) trace_store(stream, a, id, dim, it, q0);
   }
+  if (!writer) return;
   store_row((T*)a.state + row, q0, dim);
   store_row((T*)a.grad + row, g0, dim);
   if (lane == 0) ((T*)a.tlp)[crow] = (T)t0;
@@ -553,7 +584,7 @@ __global__ void __launch_bounds__(256)
   const int waves = blockDim.x >> 6;
   const int c = blockIdx.x * waves + wave;
   if (c >= n_chains) return;
-  T* scratch = (T*)smem_raw + (size_t)wave * hmc_lds_elems<T, K>();
+  T* scratch = (T*)smem_raw + (size_t)wave * hmc_lds_elems<T, K, FAM>();
   const int prob = chain_problem ? chain_problem[c] : 0;
   const hipmc_problem& pb = problems[prob];
   Target<T, K, FAM> tg;

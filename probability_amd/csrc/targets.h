@@ -28,17 +28,82 @@ __host__ __device__ inline int family_of(int kind) {
   }
 }
 
-// Loads a GLM evaluation keeps in flight per wave (one wave per chain is
-// latency bound on X, so memory-level parallelism sets its speed).
-constexpr int kGlmBatch = 16;
+// 16-byte vector of T: GLM rows are read 16 bytes per lane (one wave per
+// chain is latency bound on X, so bytes in flight per wave set its speed).
+template <typename T>
+struct Vec16;
+template <>
+struct Vec16<float> {
+  typedef float type __attribute__((ext_vector_type(4)));
+  static constexpr int n = 4;
+};
+template <>
+struct Vec16<double> {
+  typedef double type __attribute__((ext_vector_type(2)));
+  static constexpr int n = 2;
+};
+// Gradient blocks of 16 * Vec16<T>::n coefficients covering K*64.
+template <typename T, int K>
+constexpr int kGlmBlocks = (K * 64 + 16 * Vec16<T>::n - 1) / (16 * Vec16<T>::n);
 
-// v[s] for a wave-uniform slot index s (selects, no indexed registers).
+// Bytes of one half of the buffer where the W waves of one GLM chain add
+// their partial gradients [W][K*64], log-likelihoods [W][64] (double) and
+// residual sums [W][64].
+template <typename T, int K>
+__host__ __device__ constexpr size_t glm_red_bytes(int w) {
+  return (size_t)w * ((size_t)K * 64 * sizeof(T) + 64 * sizeof(double) +
+                      64 * sizeof(T));
+}
+
+// Per-wave LDS scratch of a target evaluation: GLMs need beta [K*64], the
+// residuals of a chunk [64 V] and the gradient partials [64 NB V]; the
+// other families need [K*64 + 64].
+template <typename T, int K, int FAM>
+__host__ __device__ constexpr size_t target_scratch_elems() {
+  return FAM == kFamilyGlm ? (size_t)K * 64 + 64 * Vec16<T>::n +
+                        64 * kGlmBlocks<T, K> * Vec16<T>::n
+                  : (size_t)K * 64 + 64;
+}
+
+// Bit-mask blends: v[s] for a wave-uniform slot index s, and
+// v[k] = (k == s) ? f : v[k], written with integer masks so that the
+// compiler keeps them in registers (a select chain on an index is turned
+// into a private-memory array access).
+template <typename T>
+struct Bits;
+template <>
+struct Bits<float> {
+  typedef uint32_t type;
+  static __device__ uint32_t of(float v) { return __float_as_uint(v); }
+  static __device__ float to(uint32_t b) { return __uint_as_float(b); }
+};
+template <>
+struct Bits<double> {
+  typedef uint64_t type;
+  static __device__ uint64_t of(double v) {
+    return (uint64_t)__double_as_longlong(v);
+  }
+  static __device__ double to(uint64_t b) {
+    return __longlong_as_double((long long)b);
+  }
+};
 template <typename T, int K>
 __device__ inline T pick(const T (&v)[K], int s) {
-  T r = v[0];
+  using U = typename Bits<T>::type;
+  U r = 0;
 #pragma unroll
-  for (int k = 1; k < K; ++k) r = s == k ? v[k] : r;
-  return r;
+  for (int k = 0; k < K; ++k) r |= Bits<T>::of(v[k]) & (s == k ? ~U(0) : U(0));
+  return Bits<T>::to(r);
+}
+template <typename T, int K>
+__device__ inline void put_slot(T (&v)[K], int s, T f) {
+  using U = typename Bits<T>::type;
+  const U fb = Bits<T>::of(f);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const U m = s == k ? ~U(0) : U(0);
+    v[k] = Bits<T>::to((Bits<T>::of(v[k]) & ~m) | (fb & m));
+  }
 }
 
 template <typename T, int K, int FAM>
@@ -49,6 +114,7 @@ struct Target {
   const T* __restrict__ x;   // [n_obs, dim_x] row-major (GLM)
   const T* __restrict__ xt;  // [dim_x, n_obs] (GLM)
   const T* __restrict__ y;
+  int x_pitch, xt_pitch;  // row pitches of X and X^T (elements)
   // per-lane cached data in the state layout
   T a[K], b[K];
   // bijector code of each register slot of this lane; the masks are
@@ -61,7 +127,12 @@ struct Target {
   int bslot, bcode;
   bool sp0;  // the only bijector is Softplus, all of it in slot 0
   T inv_s0sq, inv_s0, inv_s1, s1sq;  // prior constants (hier)
-  T* lds;  // per-wave scratch: GLM beta [K*64] followed by r [64]
+  T* lds;  // per-wave scratch (target_scratch_elems)
+  // GLMs: this wave's index among the waves sharing the chain, their count,
+  // and the LDS buffer where their partial sums meet (two halves, used in
+  // turn: a wave can only reuse a half after every wave has read it)
+  int split_w = 0, split_n = 1, red_half = 0;
+  unsigned char* red = nullptr;
 
   __device__ void load(const hipmc_problem& pb, T* wave_lds) {
     const int lane = lane_id();
@@ -74,6 +145,8 @@ struct Target {
     x = (const T*)pb.x;
     xt = (const T*)pb.xt;
     y = (const T*)pb.y;
+    x_pitch = pb.x_pitch;
+    xt_pitch = pb.xt_pitch;
     const T* aux = (const T*)pb.aux;
     sp_slots = 0u;
     ex_slots = 0u;
@@ -94,9 +167,9 @@ struct Target {
     one_slot = any != 0u && (any & (any - 1u)) == 0u;
     sp0 = sp_slots == 1u && ex_slots == 0u;
     bslot = any ? __builtin_ctz(any) : 0;
-    bcode = code[0];
+    bcode = 0;
 #pragma unroll
-    for (int k = 1; k < K; ++k) bcode = bslot == k ? code[k] : bcode;
+    for (int k = 0; k < K; ++k) bcode |= code[k] & (bslot == k ? ~0 : 0);
     inv_s0 = T(1) / p0;
     inv_s0sq = inv_s0 * inv_s0;
     inv_s1 = T(1) / p1;
@@ -119,6 +192,12 @@ struct Target {
         }
       }
     }
+  }
+
+  __device__ void set_split(int w, int n, unsigned char* buf) {
+    split_w = w;
+    split_n = n;
+    red = buf;
   }
 
   // Per-lane partial log-prob (double) and wave-uniform part `lpu`; writes
@@ -149,8 +228,7 @@ struct Target {
       // takes log1p(exp(x)), which equals max(x, 0) + l there.
       const T f = bcode == HIPMC_BIJ_SOFTPLUS ? fmax(xb, T(0)) + spb.l
                   : (bcode == HIPMC_BIJ_EXP ? exb : xb);
-#pragma unroll
-      for (int k = 0; k < K; ++k) th[k] = bslot == k ? f : th[k];
+      put_slot(th, bslot, f);
     } else if (sp_slots | ex_slots) {
       keep_branch();
 #pragma unroll
@@ -194,8 +272,7 @@ struct Target {
                    : (on_ex ? (double)xb : 0.0);
       const T gn = on_sp ? gb * spb.sig + spb.sig_neg
                          : (on_ex ? gb * exb + T(1) : gb);
-#pragma unroll
-      for (int k = 0; k < K; ++k) g[k] = bslot == k ? gn : g[k];
+      put_slot(g, bslot, gn);
     } else if (sp_slots | ex_slots) {
       keep_branch();
 #pragma unroll
@@ -313,75 +390,47 @@ struct Target {
   // s = X beta is computed with one row per lane (X^T reads coalesced);
   // X^T r is accumulated with one state element per lane (X reads
   // coalesced), r passing through LDS 64 rows at a time.
-  // gl[k] += sum_j X[j, k*64 + lane] r[j] over `rows` rows of row-major X
-  // (a lane per coefficient).  Unswitched on the number of live slots; the
-  // loads are unconditional (lanes past dx read column dx - 1 and are
-  // masked) so kGlmBatch (then 4, then 1) rows of loads stay in flight.
-  template <int KK>
-  __device__ void grad_rows_k(const T* xrow, int dx, int rows, const T* rbuf,
-                              T (&gl)[K]) {
-    const int lane = lane_id();
-    int col[KK];
-    bool on[KK];
+  // X^T r contribution of one chunk: lane (phase ph = lane / 16, group
+  // grp = lane % 16) adds rows ph, ph + 4, ... of the chunk, coefficients
+  // b*16V + V*grp + [0, V) of block b, read as one 16-byte vector per row
+  // and block (one load instruction covers 4 rows x 16V coefficients).
+  // Rows past `rows` are clamped and weighted by zero; blocks past the row
+  // pitch are clamped to the row start (their accumulators are never read).
+  template <int NBB>
+  __device__ void glm_grad_chunk(const T* xb, int xp, int rows, const T* rbuf,
+                                 const int (&cst)[kGlmBlocks<T, K>],
+                                 typename Vec16<T>::type (&acc)[kGlmBlocks<T, K>]) {
+    using VT = typename Vec16<T>::type;
+    constexpr int B = NBB == 1 ? 8 : 4;  // rows of loads in flight per phase
+    const int ph = lane_id() >> 4;
+    for (int j0 = 0; j0 < rows; j0 += 4 * B) {
+      VT v[B][NBB];
+      T rj[B];
 #pragma unroll
-    for (int k = 0; k < KK; ++k) {
-      const int e = k * 64 + lane;
-      on[k] = e < dx;
-      col[k] = on[k] ? e : dx - 1;
-    }
-    int j = 0;
-    for (; j + kGlmBatch <= rows; j += kGlmBatch) {
-      T v[kGlmBatch][KK];
+      for (int u = 0; u < B; ++u) {
+        const int jj = j0 + 4 * u + ph;
+        const int jl = jj < rows ? jj : rows - 1;
+        rj[u] = jj < rows ? rbuf[jj] : T(0);
 #pragma unroll
-      for (int u = 0; u < kGlmBatch; ++u)
-#pragma unroll
-        for (int k = 0; k < KK; ++k) v[u][k] = xrow[(size_t)(j + u) * dx + col[k]];
-#pragma unroll
-      for (int u = 0; u < kGlmBatch; ++u) {
-        const T rj = rbuf[j + u];
-#pragma unroll
-        for (int k = 0; k < KK; ++k) gl[k] += (on[k] ? v[u][k] : T(0)) * rj;
+        for (int b = 0; b < NBB; ++b)
+          v[u][b] = *(const VT*)(xb + (size_t)jl * xp + cst[b]);
       }
-    }
-    for (; j + 4 <= rows; j += 4) {
-      T v[4][KK];
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
+      for (int u = 0; u < B; ++u)
 #pragma unroll
-        for (int k = 0; k < KK; ++k) v[u][k] = xrow[(size_t)(j + u) * dx + col[k]];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        const T rj = rbuf[j + u];
-#pragma unroll
-        for (int k = 0; k < KK; ++k) gl[k] += (on[k] ? v[u][k] : T(0)) * rj;
-      }
-    }
-    for (; j < rows; ++j) {
-      const T rj = rbuf[j];
-#pragma unroll
-      for (int k = 0; k < KK; ++k)
-        gl[k] += (on[k] ? xrow[(size_t)j * dx + col[k]] : T(0)) * rj;
-    }
-  }
-
-  __device__ void grad_rows(const T* xrow, int dx, int rows, const T* rbuf,
-                            T (&gl)[K]) {
-    const int live = (dx + 63) >> 6;
-    if (K == 1 || live <= 1) {
-      grad_rows_k<1>(xrow, dx, rows, rbuf, gl);
-    } else if (K == 2 || live == 2) {
-      grad_rows_k<(K >= 2 ? 2 : 1)>(xrow, dx, rows, rbuf, gl);
-    } else if (K == 3 || live == 3) {
-      grad_rows_k<(K >= 3 ? 3 : 1)>(xrow, dx, rows, rbuf, gl);
-    } else {
-      grad_rows_k<K>(xrow, dx, rows, rbuf, gl);
+        for (int b = 0; b < NBB; ++b) acc[b] += v[u][b] * rj[u];
     }
   }
 
   template <bool LINEAR>
   __device__ double glm(const T (&th)[K], T (&g)[K], double& lpu) {
+    using VT = typename Vec16<T>::type;
+    constexpr int V = Vec16<T>::n;
+    constexpr int CPB = 16 * V;                 // coefficients per block
+    constexpr int NB = kGlmBlocks<T, K>;        // blocks over K*64 coefficients
     const int lane = lane_id();
     const int dx = LINEAR ? dim - 1 : dim;
+    const int n = n_obs;
     T sigma = T(1), inv_s2 = T(1);
     if (LINEAR) {
 #pragma unroll
@@ -389,8 +438,18 @@ struct Target {
         if (k == (dx >> 6)) sigma = readlane(th[k], dx & 63);
       inv_s2 = T(1) / (sigma * sigma);
     }
-    T* beta = lds;
-    T* rbuf = lds + K * 64;
+    // X and X^T rows are 16-byte aligned (pitches a multiple of 4); any
+    // other layout gives NaN rather than an unaligned read
+    if (((x_pitch | xt_pitch) & 3) != 0 || x_pitch < dx || xt_pitch < n) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) g[k] = T(NAN);
+      lpu = (double)NAN;
+      return 0.0;
+    }
+    const int xp = x_pitch, tp = xt_pitch;
+    T* beta = lds;                 // [K*64]
+    T* rbuf = lds + K * 64;        // [64V] residuals of a chunk
+    T* part = rbuf + 64 * V;       // [64 lanes][NB][V] gradient partials
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int e = k * 64 + lane;
@@ -399,62 +458,128 @@ struct Target {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    T gl[K];
+    const int grp = lane & 15;
+    int cst[NB];
+    VT acc[NB];
 #pragma unroll
-    for (int k = 0; k < K; ++k) gl[k] = T(0);
+    for (int b = 0; b < NB; ++b) {
+      const int c = b * CPB + V * grp;
+      cst[b] = c < xp ? c : 0;
+      acc[b] = VT{};
+    }
+    const int nbl = (dx + CPB - 1) / CPB;  // live blocks
     double lpl = 0.0;
     T ssq = T(0);
-    const int n = n_obs;
-    for (int n0 = 0; n0 < n; n0 += 64) {
-      const int row = n0 + lane;
-      const bool valid = row < n;
-      // s = x_row . beta: a lane per row, coalesced over X^T; rows past the
-      // end read row n - 1 and are masked, so the loads carry no branch.
-      // Batches of kGlmBatch, then 4, then 1 keep loads in flight.
-      const T* col = xt + (valid ? row : n - 1);
-      T s0 = T(0), s1 = T(0);
+    for (int n0 = split_w * 64 * V; n0 < n; n0 += split_n * 64 * V) {
+      // s for rows n0 + V*lane + [0, V): one 16-byte load of X^T per
+      // coefficient (rows past the padded end are clamped and masked)
+      const int rr = n0 + V * lane;
+      const T* col = xt + (rr < tp ? rr : tp - V);
+      VT s0 = VT{}, s1 = VT{};
       int d = 0;
-      for (; d + kGlmBatch <= dx; d += kGlmBatch) {
-        T v[kGlmBatch];
+      for (; d + 8 <= dx; d += 8) {
+        VT v[8];
 #pragma unroll
-        for (int u = 0; u < kGlmBatch; ++u) v[u] = col[(size_t)(d + u) * n];
+        for (int u = 0; u < 8; ++u) v[u] = *(const VT*)(col + (size_t)(d + u) * tp);
 #pragma unroll
-        for (int u = 0; u < kGlmBatch; u += 2) {
+        for (int u = 0; u < 8; u += 2) {
           s0 += v[u] * beta[d + u];
           s1 += v[u + 1] * beta[d + u + 1];
         }
       }
-      for (; d + 4 <= dx; d += 4) {
-        T v[4];
+      for (; d + 2 <= dx; d += 2) {
+        const VT v0 = *(const VT*)(col + (size_t)d * tp);
+        const VT v1 = *(const VT*)(col + (size_t)(d + 1) * tp);
+        s0 += v0 * beta[d];
+        s1 += v1 * beta[d + 1];
+      }
+      if (d < dx) s0 += *(const VT*)(col + (size_t)d * tp) * beta[d];
+      const VT sv = s0 + s1;
 #pragma unroll
-        for (int u = 0; u < 4; ++u) v[u] = col[(size_t)(d + u) * n];
-        s0 += v[0] * beta[d] + v[2] * beta[d + 2];
-        s1 += v[1] * beta[d + 1] + v[3] * beta[d + 3];
+      for (int v = 0; v < V; ++v) {
+        const int row = rr + v;
+        const bool valid = row < n;
+        const T s = sv[v];
+        const T yv = valid ? y[row] : T(0);
+        T r;
+        if (LINEAR) {
+          const T res = yv - s;
+          r = valid ? res * inv_s2 : T(0);
+          lpl += valid ? (double)(T(-0.5) * res * res * inv_s2) : 0.0;
+          ssq += valid ? res * res : T(0);
+        } else {
+          const T w0 = T(1) - yv;
+          const SoftplusParts<T> sp = softplus_parts(s);
+          const T l0 = w0 == T(0) ? T(0) : -(fmax(s, T(0)) + sp.l) * w0;
+          const T l1 = yv == T(0) ? T(0) : -(fmax(-s, T(0)) + sp.l) * yv;
+          lpl += valid ? (double)(l0 + l1) : 0.0;
+          r = valid ? yv - sp.sig : T(0);
+        }
+        rbuf[V * lane + v] = r;
       }
-      for (; d < dx; ++d) s0 += col[(size_t)d * n] * beta[d];
-      const T s = s0 + s1;
-      const T yv = valid ? y[row] : T(0);
-      T r;
-      if (LINEAR) {
-        const T res = yv - s;
-        r = valid ? res * inv_s2 : T(0);
-        lpl += valid ? (double)(T(-0.5) * res * res * inv_s2) : 0.0;
-        ssq += valid ? res * res : T(0);
-      } else {
-        const T w0 = T(1) - yv;
-        const SoftplusParts<T> sp = softplus_parts(s);
-        const T l0 = w0 == T(0) ? T(0) : -(fmax(s, T(0)) + sp.l) * w0;
-        const T l1 = yv == T(0) ? T(0) : -(fmax(-s, T(0)) + sp.l) * yv;
-        lpl += valid ? (double)(l0 + l1) : 0.0;
-        r = valid ? yv - sp.sig : T(0);
-      }
-      __builtin_amdgcn_wave_barrier();
-      rbuf[lane] = r;
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      const int rows = (n - n0) < 64 ? (n - n0) : 64;
-      grad_rows(x + (size_t)n0 * dx, dx, rows, rbuf, gl);
+      const int rows = (n - n0) < 64 * V ? (n - n0) : 64 * V;
+      const T* xb = x + (size_t)n0 * xp;
+      if (nbl <= 1 || NB == 1) {
+        glm_grad_chunk<1>(xb, xp, rows, rbuf, cst, acc);
+      } else if (nbl == 2 || NB == 2) {
+        glm_grad_chunk<(NB >= 2 ? 2 : 1)>(xb, xp, rows, rbuf, cst, acc);
+      } else if (nbl == 3 || NB == 3) {
+        glm_grad_chunk<(NB >= 3 ? 3 : 1)>(xb, xp, rows, rbuf, cst, acc);
+      } else {
+        glm_grad_chunk<NB>(xb, xp, rows, rbuf, cst, acc);
+      }
+      // the next chunk overwrites rbuf only after every lane has read it
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    // sum the four row phases of each coefficient in a fixed order
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+      for (int v = 0; v < V; ++v) part[(lane * NB + b) * V + v] = acc[b][v];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    T gl[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int e = k * 64 + lane;
+      const int b = e / CPB, gi = (e % CPB) / V, vi = e % V;
+      T sum = T(0);
+#pragma unroll
+      for (int p = 0; p < 4; ++p) sum += part[((p * 16 + gi) * NB + b) * V + vi];
+      gl[k] = sum;
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (split_n > 1) {
+      // every wave adds the waves' partials in the same order, so all of
+      // them continue with identical sums
+      unsigned char* half = red + (size_t)red_half * glm_red_bytes<T, K>(split_n);
+      T* rg = (T*)half;
+      double* rl = (double*)(rg + (size_t)split_n * K * 64);
+      T* rs = (T*)(rl + (size_t)split_n * 64);
+#pragma unroll
+      for (int k = 0; k < K; ++k) rg[(split_w * K + k) * 64 + lane] = gl[k];
+      rl[split_w * 64 + lane] = lpl;
+      rs[split_w * 64 + lane] = ssq;
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < K; ++k) gl[k] = T(0);
+      lpl = 0.0;
+      ssq = T(0);
+      for (int w = 0; w < split_n; ++w) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) gl[k] += rg[(w * K + k) * 64 + lane];
+        lpl += rl[w * 64 + lane];
+        ssq += rs[w * 64 + lane];
+      }
+      red_half ^= 1;
     }
     double lp = log_norm;
 #pragma unroll

@@ -53,6 +53,9 @@ class Run:
     seed = np.asarray(seed, np.uint32).reshape(2)
     a.seed[0], a.seed[1] = int(seed[0]), int(seed[1])
     a.target_kind = spec.kind
+    if spec.kind in (_lib.TARGET_LOGISTIC, _lib.TARGET_LINEAR):
+      base = getattr(spec, '_base', spec)
+      a.waves_per_chain = _lib.glm_waves_per_chain(base.n_obs * base.dim)
     a.max_energy_diff = 1000.0
     a.problems = table.data_ptr()
     a.n_problems = 1

@@ -53,10 +53,13 @@ class PosteriorBatch:
       per lane, chosen from the largest dim of the launch; passing the same
       slot_dim to every batch a problem may run in makes it run the same
       code (bit-identical results) alone or in any mix.
+    waves_per_chain: GLMs only, waves sharing one chain's gradient (1-4;
+      None: chosen from the largest design matrix).  Like slot_dim, equal
+      values give bit-identical results alone or in any mix.
   """
 
   def __init__(self, specs, num_chains, bijectors=None, keys=None,
-               slot_dim=None):
+               slot_dim=None, waves_per_chain=None):
     if not specs:
       raise ValueError('need at least one problem')
     self.specs = [targets_lib.as_target(s) for s in specs]
@@ -94,6 +97,13 @@ class PosteriorBatch:
     self.kind = self.specs[0].kind
     if _family(self.kind) == 2 and len({s.kind for s in self.specs}) > 1:
       self.kind = _lib.TARGET_DIAG_NORMAL
+    if _family(self.kind) != 1:
+      self.waves_per_chain = 1
+    elif waves_per_chain is not None:
+      self.waves_per_chain = int(waves_per_chain)
+    else:
+      self.waves_per_chain = _lib.glm_waves_per_chain(
+          max(s.n_obs * s.dim for s in self.specs))
 
   @property
   def n_problems(self):
@@ -195,6 +205,7 @@ def sample_posteriors(batch, num_results, num_burnin_steps=0,
     a.max_tree_depth = int(max_tree_depth)
     a.unrolled_leapfrog_steps = 1
     a.target_kind = batch.kind
+    a.waves_per_chain = batch.waves_per_chain
     a.max_energy_diff = float(max_energy_diff)
     a.problems = batch.table.data_ptr()
     a.n_problems = p

@@ -6,10 +6,12 @@ free.  Admitted requests keep their chain state, step size and
 dual-averaging accumulators resident in device pools; every scheduling tick
 launches, per target family, one NUTS launch over all requests still adapting
 (one transition, then a per-request dual-averaging update) and one over all
-requests in their sampling phase (several transitions), on two streams so
-that the two overlap.  A request's chains are keyed by (its seed, its key,
-chain index) and advance by per-request iteration counters, so its draws
-do not depend on which other requests share the launches.
+requests in their sampling phase (several transitions).  Each (family,
+phase) pair is a lane with its own stream that advances as soon as its own
+previous launch has finished, so short requests are not held back by the
+slowest chains of another lane.  A request's chains are keyed by (its seed,
+its key, chain index) and advance by per-request iteration counters, so its
+draws do not depend on which other requests share the launches.
 
 Per-request semantics: sample_chain(DualAveraging(TTK(NUTS))) with
 num_burnin_steps = num_adaptation_steps (`mcmc/sample.py`, `mcmc/nuts.py`,
@@ -61,6 +63,7 @@ class Request:
     self.final_step_size = None
     self.rows = None
     self.da_slot = None
+    self.in_flight = False
 
   @property
   def total_steps(self):
@@ -135,6 +138,17 @@ class _Pool:
     req.rows = None
 
 
+class _Lane:
+  """The launches of one (family, phase): a stream and the launch in flight
+  (completion event, its requests, buffers kept alive until it ends)."""
+
+  def __init__(self, family, adapting, device):
+    self.family = family
+    self.adapting = adapting
+    self.stream = torch.cuda.Stream(device=device)
+    self.inflight = None
+
+
 class SamplerServer:
   """Admits requests by HBM budget and advances them in shared launches.
 
@@ -144,25 +158,28 @@ class SamplerServer:
     iters_per_tick: transitions a sampling-phase request makes per launch.
     slot_dims: register-slot class per family ({'glm': 192, 'hier': 512});
       every request of a family runs the same kernel code.
+    glm_waves_per_chain: waves sharing each GLM chain's gradient (fixed per
+      server, so a request's draws do not depend on the mix).
   """
 
   def __init__(self, dtype=torch.float32, device='cuda',
                hbm_budget_bytes=32 << 30, max_chains=32768, max_requests=2048,
-               iters_per_tick=8, slot_dims=None):
+               iters_per_tick=8, slot_dims=None, glm_waves_per_chain=4):
     _lib.require_cuda()
     self.dtype = dtype
     self.device = torch.device(device)
     self.budget = int(hbm_budget_bytes)
     self.used = 0
     self.iters_per_tick = int(iters_per_tick)
+    self.glm_waves_per_chain = int(glm_waves_per_chain)
     slot_dims = slot_dims or {'glm': 192, 'hier': 512}
     self.pools = {f: _Pool(f, slot_dims[f], max_chains, max_requests, dtype,
                            self.device) for f in slot_dims}
     self.queue = collections.deque()
     self.active = []
     self.finished = []
-    self.streams = [torch.cuda.Stream(device=self.device),
-                    torch.cuda.Stream(device=self.device)]
+    self.lanes = [_Lane(f, ad, self.device) for f in slot_dims
+                  for ad in (True, False)]
     self.ticks = 0
 
   # ------------------------------------------------------------ admission
@@ -253,6 +270,8 @@ class SamplerServer:
       a.unrolled_leapfrog_steps = 1
       a.target_kind = (_lib.TARGET_HIER_NORMAL if pool.family == 'hier'
                        else _lib.TARGET_LOGISTIC)
+      if pool.family == 'glm':
+        a.waves_per_chain = self.glm_waves_per_chain
       a.max_energy_diff = 1000.0
       a.problems = table.data_ptr()
       a.n_problems = p
@@ -303,24 +322,54 @@ class SamplerServer:
         _lib.ptr(pool.step64), stream.cuda_stream))
 
   def tick(self):
-    """Admits what fits, advances every active request once, retires the
-    finished ones.  Returns the requests finished in this tick."""
-    self._admit()
-    keep = []
-    for fam, pool in self.pools.items():
-      reqs = [r for r in self.active if _family(r.spec) == fam]
-      ad = [r for r in reqs if r.adapting]
-      sm = [r for r in reqs if not r.adapting]
-      keep.append(self._launch(pool, ad, True, self.streams[0]))
-      keep.append(self._launch(pool, sm, False, self.streams[1]))
-    for s in self.streams:
-      s.synchronize()
-    del keep
-    self.ticks += 1
+    """Admits what fits, then for every lane whose launch has finished
+    advances its requests, retires the finished ones and starts the lane's
+    next launch; waits for the oldest launch when none has finished.
+    Returns the requests finished in this call."""
+    if self._admit():
+      # the admitted rows were written on the current stream
+      ev = torch.cuda.Event()
+      ev.record()
+      for lane in self.lanes:
+        lane.stream.wait_event(ev)
     done = []
+    while True:
+      progressed = False
+      for lane in self.lanes:
+        if lane.inflight is not None:
+          if not lane.inflight[0].query():
+            continue
+          done += self._complete(lane)
+          progressed = True
+        progressed |= self._start(lane)
+      busy = [l for l in self.lanes if l.inflight is not None]
+      if progressed or not busy:
+        return done
+      busy[0].inflight[0].synchronize()
+
+  def _start(self, lane):
+    reqs = [r for r in self.active if _family(r.spec) == lane.family and
+            r.adapting == lane.adapting and not r.in_flight]
+    if not reqs:
+      return False
+    pool = self.pools[lane.family]
+    keep = self._launch(pool, reqs, lane.adapting, lane.stream)
+    ev = torch.cuda.Event()
+    ev.record(lane.stream)
+    for r in reqs:
+      r.in_flight = True
+    lane.inflight = (ev, reqs, keep)
+    return True
+
+  def _complete(self, lane):
+    _, reqs, _ = lane.inflight
+    lane.inflight = None
+    self.ticks += 1
     now = time.perf_counter()
-    for r in self.active:
-      if r.adapting:
+    done = []
+    for r in reqs:
+      r.in_flight = False
+      if lane.adapting:
         r.iter += 1
       else:
         r.iter = min(r.total_steps, r.iter + self.iters_per_tick)

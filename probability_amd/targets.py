@@ -215,7 +215,30 @@ def unpack_state(flat, chain_shape, event_shapes):
   return out
 
 
-class LogisticRegression(TargetSpec):
+def _padded_rows(a, pitch_multiple=4):
+  """(storage [r, pitch], view [r, c]) of a 2-D tensor whose rows start
+  16-byte aligned: the pitch is c rounded up to `pitch_multiple` and the
+  padding is zero."""
+  r, c = a.shape
+  pitch = -(-c // pitch_multiple) * pitch_multiple
+  buf = torch.zeros((r, pitch), dtype=a.dtype, device=a.device)
+  buf[:, :c] = a
+  return buf, buf[:, :c]
+
+
+class _GlmData:
+  """X stored row-major and transposed, each with padded 16-byte rows."""
+
+  def _set_design(self, x):
+    self._xbuf, self.x = _padded_rows(x)
+    self._xtbuf, self.xt = _padded_rows(x.t())
+
+  def _fill_design(self, p):
+    p.x, p.xt = self._xbuf.data_ptr(), self._xtbuf.data_ptr()
+    p.x_pitch, p.xt_pitch = self._xbuf.shape[1], self._xtbuf.shape[1]
+
+
+class LogisticRegression(_GlmData, TargetSpec):
   """Bayesian logistic regression.
 
   beta ~ N(0, prior_scale^2 I_D); y_n ~ Bernoulli(logits = x_n . beta).
@@ -226,8 +249,7 @@ class LogisticRegression(TargetSpec):
 
   def __init__(self, x, y, prior_scale=1.0, dtype=torch.float32, device=None):
     super().__init__(dtype, device)
-    self.x = _to_device(x, dtype, self.device)
-    self.xt = self.x.t().contiguous()
+    self._set_design(_to_device(x, dtype, self.device))
     self.y = _to_device(y, dtype, self.device).reshape(-1)
     self.n_obs, d = self.x.shape
     if self.y.shape[0] != self.n_obs:
@@ -238,7 +260,8 @@ class LogisticRegression(TargetSpec):
   def _fill(self, p):
     d = self.dim
     p.n_obs = self.n_obs
-    p.x, p.xt, p.y = self.x.data_ptr(), self.xt.data_ptr(), self.y.data_ptr()
+    self._fill_design(p)
+    p.y = self.y.data_ptr()
     p.param[0] = self.prior_scale
     p.log_norm = d * (-_HALF_LOG_2PI - math.log(self.prior_scale))
 
@@ -275,7 +298,7 @@ class LogisticRegressionBF16(LogisticRegression):
     return super().log_prob_and_grad_flat(x, bijectors)
 
 
-class LinearRegression(TargetSpec):
+class LinearRegression(_GlmData, TargetSpec):
   """Bayesian Gaussian linear regression with unknown noise scale.
 
   beta ~ N(0, prior_scale^2 I_D); sigma ~ HalfNormal(sigma_scale);
@@ -287,8 +310,7 @@ class LinearRegression(TargetSpec):
   def __init__(self, x, y, prior_scale=1.0, sigma_scale=1.0,
                dtype=torch.float32, device=None):
     super().__init__(dtype, device)
-    self.x = _to_device(x, dtype, self.device)
-    self.xt = self.x.t().contiguous()
+    self._set_design(_to_device(x, dtype, self.device))
     self.y = _to_device(y, dtype, self.device).reshape(-1)
     self.n_obs, d = self.x.shape
     self.prior_scale = float(prior_scale)
@@ -298,7 +320,8 @@ class LinearRegression(TargetSpec):
   def _fill(self, p):
     dx = self.dim - 1
     p.n_obs = self.n_obs
-    p.x, p.xt, p.y = self.x.data_ptr(), self.xt.data_ptr(), self.y.data_ptr()
+    self._fill_design(p)
+    p.y = self.y.data_ptr()
     p.param[0] = self.prior_scale
     p.param[1] = self.sigma_scale
     p.log_norm = (self.n_obs * -_HALF_LOG_2PI

@@ -52,10 +52,12 @@ def _run(batch, seed=3):
 def test_batched_equals_each_problem_alone(dtype):
   specs, bij, _ = _problems(dtype, n=5)
   chains = [8, 5, 8, 3, 6]
-  mix = _run(batched.PosteriorBatch(specs, chains, bij, slot_dim=128))
+  mix = _run(batched.PosteriorBatch(specs, chains, bij, slot_dim=128,
+                                    waves_per_chain=2))
   for k in range(len(specs)):
     alone = _run(batched.PosteriorBatch([specs[k]], [chains[k]], [bij[k]],
-                                        keys=[k], slot_dim=128))
+                                        keys=[k], slot_dim=128,
+                                        waves_per_chain=2))
     np.testing.assert_array_equal(mix.draws[k].cpu().numpy(),
                                   alone.draws[0].cpu().numpy())
     np.testing.assert_array_equal(mix.leapfrogs_taken[k].cpu().numpy(),

@@ -41,6 +41,24 @@ def _case(kind, dtype_np, seed=0, c=64):
     mk = lambda dt: ptg.LogisticRegression(x, y, 1.0, dtype=dt)
     q = cast(r.normal(size=(c, 7)) * 0.3)
     return o, mk, q, None
+  if kind == 'logistic_split':
+    # 2051 x 9: big enough that four waves share each chain's gradient
+    x = cast(r.normal(size=(2051, 9)) * 0.3)
+    y = (r.uniform(size=2051) < 0.5).astype(np.float64)
+    o = otg.LogisticRegression(x, y, 1.0)
+    mk = lambda dt: ptg.LogisticRegression(x, y, 1.0, dtype=dt)
+    q = cast(r.normal(size=(c, 9)) * 0.1)
+    return o, mk, q, None
+  if kind == 'linear_split':
+    x = cast(r.normal(size=(1500, 11)))
+    y = cast(x @ r.normal(size=11) * 0.3 + 0.5 * r.normal(size=1500))
+    o = otg.LinearRegression(x, y, 1.0, 1.0)
+    mk = lambda dt: ptg.LinearRegression(x, y, 1.0, 1.0, dtype=dt)
+    q = cast(np.concatenate([r.normal(size=(c, 11)) * 0.1,
+                             np.full((c, 1), -0.5)], 1))
+    codes = np.zeros(12, np.int32)
+    codes[11] = 2  # Exp on sigma
+    return o, mk, q, codes
   if kind == 'linear':
     x = cast(r.normal(size=(150, 6)))
     y = cast(x @ r.normal(size=6) + 0.5 * r.normal(size=150))
@@ -132,7 +150,7 @@ def test_draws_match_oracle(tdt, ndt, tol):
 # ---------------------------------------------------------------- targets
 @pytest.mark.parametrize('tdt,ndt,tol', DTYPES)
 @pytest.mark.parametrize('kind', ['logistic', 'linear', 'hier', 'diag',
-                                  'funnel'])
+                                  'funnel', 'logistic_split', 'linear_split'])
 def test_target_eval_matches_oracle(kind, tdt, ndt, tol):
   o, mk, q, codes = _case(kind, ndt)
   spec = mk(tdt)
@@ -158,12 +176,13 @@ def test_target_callable_like_tfp(tdt, ndt, tol):
 
 # ---------------------------------------------------------------- HMC
 @pytest.mark.parametrize('tdt,ndt,tol', DTYPES)
-@pytest.mark.parametrize('kind', ['logistic', 'hier'])
+@pytest.mark.parametrize('kind', ['logistic', 'hier', 'logistic_split'])
 def test_hmc_one_step_matches_oracle(kind, tdt, ndt, tol):
   o, mk, q, codes = _case(kind, ndt, seed=3)
   spec = mk(tdt)
   ot = _otarget(o, codes)
-  eps, nl = (0.05, 10) if kind == 'logistic' else (0.2, 6)
+  eps, nl = {'logistic': (0.05, 10), 'hier': (0.2, 6),
+             'logistic_split': (0.05, 8)}[kind]
   kern = pmcmc.HamiltonianMonteCarlo(
       spec if codes is None else ptg.TransformedTarget(spec, _bijs(spec, codes)),
       step_size=eps, num_leapfrog_steps=nl)
@@ -217,7 +236,8 @@ def _nuts_one(kind, tdt, ndt, seed=0, c=64, eps=None, max_depth=10):
   spec = mk(tdt)
   ot = _otarget(o, codes)
   eps = eps or {'logistic': 0.08, 'linear': 0.05, 'hier': 0.3, 'diag': 0.5,
-                'funnel': 0.9}[kind]
+                'funnel': 0.9, 'logistic_split': 0.04,
+                'linear_split': 0.02}[kind]
   tspec = spec if codes is None else ptg.TransformedTarget(spec,
                                                            _bijs(spec, codes))
   kern = pmcmc.NoUTurnSampler(tspec, step_size=np.asarray(eps, ndt),
@@ -236,7 +256,7 @@ def _nuts_one(kind, tdt, ndt, seed=0, c=64, eps=None, max_depth=10):
 
 @pytest.mark.parametrize('tdt,ndt,tol', DTYPES)
 @pytest.mark.parametrize('kind', ['hier', 'logistic', 'linear', 'diag',
-                                  'funnel'])
+                                  'funnel', 'logistic_split', 'linear_split'])
 def test_nuts_one_step_matches_oracle(kind, tdt, ndt, tol):
   for seed in range(3):
     spec, nxt, res, oq, ores = _nuts_one(kind, tdt, ndt, seed=seed)
