@@ -75,8 +75,9 @@ def parse():
   ap.add_argument('--plain-sample', type=int, default=16,
                   help='ragged GLMs: problems timed one per launch')
   ap.add_argument('--glm-waves', type=int, default=None,
-                  help='ragged GLMs: waves sharing each chain (default: by '
-                       'problem size)')
+                  help='ragged GLMs: one batch with this many waves '
+                       'sharing each chain (default: size classes, each '
+                       'problem by its own N x D, on concurrent streams)')
   return ap.parse_args()
 
 

@@ -238,28 +238,46 @@ def _ragged_specs(torch, pa, n_problems):
 
 
 def ragged_glms_512(args, torch, pa):
+  from probability_amd import _lib
   from probability_amd.mcmc import batched
   p, chains = 512, 32
   na = 500 if args.adapt is None else args.adapt
   nr = 1000 if args.draws is None else args.draws
   specs, bij = _ragged_specs(torch, pa, p)
   slot = 192
-  batch = batched.PosteriorBatch(specs, chains, bijectors=bij, slot_dim=slot,
-                                 waves_per_chain=args.glm_waves)
   # the sampling phase runs 100 transitions per launch so that progress is
   # visible between launches
   kw = dict(num_results=nr, num_burnin_steps=na, num_adaptation_steps=na,
             step_size=0.05, target_accept_prob=0.8, max_tree_depth=10,
             max_iters_per_launch=100)
+  mark = [0]
 
   def report(t, n):
-    if t % 25 == 0 or t == n:
+    if t >= mark[0] + 100 or t == n:
+      mark[0] = 0 if t == n else t
       torch.cuda.synchronize()
       _say(f'ragged: {t}/{n} transitions')
 
-  def run(k, stats=None):
-    return batched.sample_posteriors(batch, seed=args.seed + k, stats=stats,
-                                     progress=report, **kw)
+  if args.glm_waves is None:
+    # size classes: each problem's chains are shared by the waves its own
+    # N x D asks for; the classes run concurrently on their own streams
+    def run(k, stats=None):
+      return batched.sample_ragged(specs, chains, bijectors=bij,
+                                   slot_dim=slot, seed=args.seed + k,
+                                   stats=stats, progress=report, **kw)
+
+    def waves_of(i):
+      return _lib.glm_waves_per_chain(specs[i].n_obs * specs[i].dim)
+  else:
+    batch = batched.PosteriorBatch(specs, chains, bijectors=bij,
+                                   slot_dim=slot, waves_per_chain=args.glm_waves)
+
+    def run(k, stats=None):
+      return batched.sample_posteriors(batch, seed=args.seed + k, stats=stats,
+                                       progress=report, **kw)
+
+    def waves_of(i):
+      return batch.waves_per_chain
 
   _say('ragged: batched runs')
   stats = {'time_kernels': True}
@@ -283,12 +301,14 @@ def ragged_glms_512(args, torch, pa):
     _say(f'ragged: problem {i} alone')
     one = batched.PosteriorBatch([specs[i]], chains, bijectors=[bij[i]],
                                  keys=[i], slot_dim=slot,
-                                 waves_per_chain=batch.waves_per_chain)
+                                 waves_per_chain=waves_of(i))
     _, pm = _timed(torch, lambda k: batched.sample_posteriors(
         one, seed=args.seed + k, progress=report, **kw), 1, 0)
     plain_ms.append(pm)
   plain_total_ms = float(np.mean(plain_ms)) * p
   agg = float(min_ess.sum()) / (ms / 1e3)
+  classes = stats.get('classes', [(p, waves_of(0))])
+  cls_stats = stats.get('class_stats', [stats])
   line = _base('aggregate min-ESS/sec over 512 ragged GLM posteriors (NUTS+DA, '
                '32 chains each), batched vs one posterior per launch',
                agg, 'min-ESS/s', args, ms, 'f32',
@@ -297,7 +317,18 @@ def ragged_glms_512(args, torch, pa):
                             '32 chains, NUTS(depth 10) + per-problem DA '
                             f'(target 0.8), {na} adaptation + {nr} draws',
                 'problems': p, 'chains_per_problem': chains,
-                'waves_per_chain': batch.waves_per_chain,
+                'size_classes': [
+                    {'problems': n, 'waves_per_chain': w,
+                     'entries_total': cs['cost_total'],
+                     'entries_adaptation': cs['cost_adaptation'],
+                     'entries_critical_path': cs['cost_critical'],
+                     'entries_critical_path_adaptation':
+                         cs['cost_critical_adaptation'],
+                     'adaptation_kernel_ms': sum(
+                         l[0] for l in cs['launches'] if l[2] == 1),
+                     'sampling_kernel_ms': sum(
+                         l[0] for l in cs['launches'] if l[2] > 1)}
+                    for (n, w), cs in zip(classes, cls_stats)],
                 'sum_n_times_d': nd})
   line.update({
       'min_ess_per_problem_min_median': [float(min_ess.min()),
@@ -310,23 +341,22 @@ def ragged_glms_512(args, torch, pa):
                 'what': f'each problem alone in its own launches '
                         f'({len(idx)} problems timed, mean x 512)'},
       'batched_over_plain': plain_total_ms / ms,
-      'roofline': _ragged_roofline(stats, specs, chains, lf_total)})
+      'roofline': _ragged_roofline(stats, specs, lf_total, ms)})
   return line
 
 
-def _ragged_roofline(stats, specs, chains, lf_total):
-  kern_ms = sum(l[0] for l in stats['launches'])
-  adapt_ms = sum(l[0] for l in stats['launches'] if l[2] == 1)
+def _ragged_roofline(stats, specs, lf_total, wall_ms):
   # algorithmic FLOP: 4 N_k D_k per chain-leapfrog, weighted by each
-  # problem's share of the leapfrogs (per-problem totals)
+  # problem's share of the leapfrogs (per-problem totals), over the whole
+  # run's wall time (the size classes' kernels overlap on their streams)
   lf = stats['leapfrogs_per_problem']
   flop = float(sum(4.0 * s.n_obs * s.dim * float(lf[i])
                    for i, s in enumerate(specs)))
-  achieved = flop / (kern_ms * 1e-3) / 1e12
+  achieved = flop / (wall_ms * 1e-3) / 1e12
   return {'bound': 'mfma', 'achieved': achieved, 'peak': FP32_PEAK_TFLOPS,
           'unit': 'TFLOP/s', 'frac': achieved / FP32_PEAK_TFLOPS,
-          'traffic': None, 'kernel': 'nuts_kernel<float,3,glm>',
-          'kernel_ms': kern_ms, 'adaptation_kernel_ms': adapt_ms,
+          'traffic': None, 'kernel': 'nuts_kernel<float,3,glm> (all size '
+                                     'classes, wall time)',
           'chain_leapfrogs': lf_total}
 
 

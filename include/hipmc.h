@@ -174,7 +174,9 @@ typedef struct hipmc_run_args {
   /* LOGISTIC / LINEAR launches of hipmc_nuts_run / hipmc_hmc_run: waves that
    * share one chain's gradient (each takes every waves_per_chain-th block of
    * rows; partial sums meet in LDS).  0 or 1: one wave per chain.  At most
-   * 4; other target kinds ignore it. */
+   * 8 for hipmc_nuts_run and 4 for hipmc_hmc_run (larger values are
+   * clamped), and fewer where the chain's LDS would exceed 160 KiB (large
+   * slot counts in FP64); other target kinds ignore it. */
   int32_t waves_per_chain;
   const int32_t* chain_problem;
   const int32_t* chain_local;

@@ -170,9 +170,11 @@ def stream_handle(device=None):
 
 def glm_waves_per_chain(max_rows_times_cols):
   """Waves that share one GLM chain's gradient for a launch whose largest
-  design matrix has `max_rows_times_cols` entries: one wave streams X at
-  about one 16-byte load per lane in flight, so big problems split their
-  rows over up to four waves."""
+  design matrix has `max_rows_times_cols` entries: a wave's reads of X are
+  bounded by the loads it keeps in flight, so big problems split their rows
+  over up to eight waves (one chain per CU)."""
+  if max_rows_times_cols >= 262144:
+    return 8
   if max_rows_times_cols >= 16384:
     return 4
   if max_rows_times_cols >= 4096:

@@ -40,6 +40,20 @@ int hip_check(hipError_t e, const char* what) {
 hipStream_t as_stream(uint64_t s) { return reinterpret_cast<hipStream_t>(s); }
 
 constexpr int kMaxLdsPerBlock = 64 * 1024;
+// gfx950 gives one workgroup up to 160 KiB of LDS; a launch that asks for
+// more than 64 KiB raises the kernel's dynamic-LDS limit first.
+constexpr int kMaxLdsSharedChain = 160 * 1024;
+
+template <typename F>
+int allow_lds(F* kernel, size_t bytes) {
+  if (bytes > (size_t)kMaxLdsSharedChain)
+    return fail(HIPMC_E_UNSUPPORTED, "shared-chain LDS does not fit");
+  if (bytes <= (size_t)kMaxLdsPerBlock) return HIPMC_OK;
+  return hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)bytes),
+                   "hipFuncSetAttribute(max dynamic LDS)");
+}
 
 // Smallest compiled slot count K with K*64 >= max_dim.
 int pick_k(int max_dim) {
@@ -86,18 +100,24 @@ int check_run(const hipmc_run_args* a, bool nuts) {
 }
 
 template <typename T, int K, int FAM>
-int launch_nuts(const hipmc_run_args& a, hipStream_t s) {
-  if (a.n_chains == 0) return HIPMC_OK;
-  const int W = waves_per_chain<FAM>(a);
+int launch_nuts(const hipmc_run_args& a0, hipStream_t s) {
+  if (a0.n_chains == 0) return HIPMC_OK;
+  // one chain per block: one U-turn memory, W target scratches and the two
+  // halves of the partial-sum buffer; fewer waves share the chain when that
+  // does not fit
+  auto shared_bytes = [&](int w) {
+    return ((size_t)2 * a0.max_tree_depth * K * 64 +
+            (size_t)w * target_scratch_elems<T, K, FAM>()) * sizeof(T) +
+           2 * glm_red_bytes<T, K>(w);
+  };
+  hipmc_run_args a = a0;
+  int W = waves_per_chain<FAM>(a);
+  while (W > 1 && shared_bytes(W) > (size_t)kMaxLdsSharedChain) --W;
+  a.waves_per_chain = W;
   if (W > 1) {
-    // one chain per block: one U-turn memory, W target scratches and the
-    // two halves of the partial-sum buffer
-    const size_t bytes =
-        ((size_t)2 * a.max_tree_depth * K * 64 +
-         (size_t)W * target_scratch_elems<T, K, FAM>()) * sizeof(T) +
-        2 * glm_red_bytes<T, K>(W);
-    if (bytes > (size_t)kMaxLdsPerBlock)
-      return fail(HIPMC_E_UNSUPPORTED, "shared-chain LDS does not fit");
+    const size_t bytes = shared_bytes(W);
+    const int rc = allow_lds(&nuts_kernel<T, K, FAM>, bytes);
+    if (rc != HIPMC_OK) return rc;
     hipLaunchKernelGGL((nuts_kernel<T, K, FAM>), dim3(a.n_chains),
                        dim3(64 * W), bytes, s, a);
     return hip_check(hipGetLastError(), "nuts_kernel launch");
@@ -113,12 +133,23 @@ int launch_nuts(const hipmc_run_args& a, hipStream_t s) {
 }
 
 template <typename T, int K, int FAM, bool CAL>
-int launch_hmc(const hipmc_run_args& a, hipStream_t s) {
-  if (a.n_chains == 0) return HIPMC_OK;
+int launch_hmc(const hipmc_run_args& a0, hipStream_t s) {
+  if (a0.n_chains == 0) return HIPMC_OK;
   const size_t per_wave = hmc_lds_elems<T, K, FAM>() * sizeof(T);
-  const int W = waves_per_chain<FAM>(a);
+  hipmc_run_args a = a0;
+  // hmc_kernel is built for blocks of at most 256 threads (it may use more
+  // than 256 registers per lane, one wave per SIMD), so a shared chain
+  // spans at most four waves
+  int W = waves_per_chain<FAM>(a);
+  if (W > 4) W = 4;
+  while (W > 1 && per_wave * W + 2 * glm_red_bytes<T, K>(W) >
+                      (size_t)kMaxLdsSharedChain)
+    --W;
+  a.waves_per_chain = W;
   if (W > 1) {
     const size_t bytes = per_wave * W + 2 * glm_red_bytes<T, K>(W);
+    const int rc = allow_lds(&hmc_kernel<T, K, FAM, CAL>, bytes);
+    if (rc != HIPMC_OK) return rc;
     hipLaunchKernelGGL((hmc_kernel<T, K, FAM, CAL>), dim3(a.n_chains),
                        dim3(64 * W), bytes, s, a);
     return hip_check(hipGetLastError(), "hmc_kernel launch");

@@ -213,11 +213,12 @@ __device__ inline void put(void* base, size_t i, T v) {
 }
 
 
-// Waves sharing one chain: only GLM launches split a chain (at most 4).
+// Waves sharing one chain: only GLM launches split a chain (at most 8,
+// which at two waves per SIMD is one chain per CU).
 template <int FAM>
 __host__ __device__ inline int waves_per_chain(const hipmc_run_args& a) {
   if (FAM != kFamilyGlm || a.waves_per_chain <= 1) return 1;
-  return a.waves_per_chain < 4 ? a.waves_per_chain : 4;
+  return a.waves_per_chain < 8 ? a.waves_per_chain : 8;
 }
 
 // ------------------------------------------------------------------ NUTS
@@ -229,10 +230,11 @@ __host__ __device__ constexpr size_t nuts_lds_elems(int max_depth) {
 }
 
 // GLM chains are bound by loads in flight, so their kernel is built for two
-// waves per SIMD (the register budget that allows); the other families keep
-// one wave per SIMD and every register for the leaf's latency.
+// waves per SIMD (the register budget that allows) and blocks of up to
+// eight waves (one chain shared by a whole CU); the other families keep one
+// wave per SIMD and every register for the leaf's latency.
 template <typename T, int K, int FAM>
-__global__ void __launch_bounds__(256)
+__global__ void __launch_bounds__(FAM == kFamilyGlm ? 512 : 256)
     __attribute__((amdgpu_waves_per_eu(FAM == kFamilyGlm ? 2 : 1)))
     nuts_kernel(hipmc_run_args a) {
   extern __shared__ __align__(16) unsigned char smem_raw[];

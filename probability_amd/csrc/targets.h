@@ -385,43 +385,151 @@ struct Target {
     return lpl;
   }
 
+  // The row loop of one GLM evaluation; this wave handles chunks of 64V
+  // rows n0 = split_w * 64V, advancing by split_n * 64V.
+  //  s-pass: lane owns rows n0 + V*lane + [0, V) and reads one 16-byte
+  //   vector of X^T per coefficient, 8 coefficients per batch; the next
+  //   batch is issued before the current one is used, and coefficients past
+  //   dx read the last column and meet beta = 0 (exact zero terms), so the
+  //   sums equal the plain even/odd split.
+  //  gradient pass: lane (phase ph = lane / 16, group grp = lane % 16) adds
+  //   rows ph, ph + 4, ... of the chunk, coefficients b*16V + V*grp + [0, V)
+  //   of block b (one load instruction covers 4 rows x 16V coefficients);
+  //   B rows per phase per batch, the next batch in flight while the current
+  //   one is used, and the first batch issued before the residuals are
+  //   computed.  Rows past the chunk are clamped and weighted by zero;
+  //   blocks past the row pitch are clamped to the row start (their
+  //   accumulators are never read).
+  template <int NBB, bool LINEAR>
+  __device__ void glm_rows(const T* beta, T* rbuf, int dx, int xp, int tp,
+                           T inv_s2, const int (&cst)[kGlmBlocks<T, K>],
+                           typename Vec16<T>::type (&acc)[kGlmBlocks<T, K>],
+                           double& lpl, T& ssq) {
+    using VT = typename Vec16<T>::type;
+    constexpr int V = Vec16<T>::n;
+    constexpr int SB = 8;
+    constexpr int B = NBB == 1 ? 8 : 4;
+    const int lane = lane_id();
+    const int ph = lane >> 4;
+    const int n = n_obs;
+    const int nsb = (dx + SB - 1) / SB;
+    const int dlast = dx - 1;
+    for (int n0 = split_w * 64 * V; n0 < n; n0 += split_n * 64 * V) {
+      const int rr = n0 + V * lane;
+      const T* col = xt + (rr < tp ? rr : tp - V);
+      T yv[V];
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        const int row = rr + v;
+        yv[v] = y[row < n ? row : n - 1];
+      }
+      VT cur[SB];
+#pragma unroll
+      for (int u = 0; u < SB; ++u)
+        cur[u] = *(const VT*)(col + (size_t)(u < dlast ? u : dlast) * tp);
+      VT s0 = VT{}, s1 = VT{};
+      for (int bi = 0; bi + 1 < nsb; ++bi) {
+        const int d = bi * SB;
+        VT nxt[SB];
+#pragma unroll
+        for (int u = 0; u < SB; ++u) {
+          const int dd = d + SB + u;
+          nxt[u] = *(const VT*)(col + (size_t)(dd < dlast ? dd : dlast) * tp);
+        }
+#pragma unroll
+        for (int u = 0; u < SB; u += 2) {
+          s0 += cur[u] * beta[d + u];
+          s1 += cur[u + 1] * beta[d + u + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < SB; ++u) cur[u] = nxt[u];
+      }
+      {
+        const int d = (nsb - 1) * SB;
+#pragma unroll
+        for (int u = 0; u < SB; u += 2) {
+          s0 += cur[u] * beta[d + u];
+          s1 += cur[u + 1] * beta[d + u + 1];
+        }
+      }
+      const VT sv = s0 + s1;
+      const int rows = (n - n0) < 64 * V ? (n - n0) : 64 * V;
+      const T* xb = x + (size_t)n0 * xp;
+      VT gv[B][NBB];
+#pragma unroll
+      for (int u = 0; u < B; ++u) {
+        const int jj = 4 * u + ph;
+        const int jl = jj < rows ? jj : rows - 1;
+#pragma unroll
+        for (int b = 0; b < NBB; ++b)
+          gv[u][b] = *(const VT*)(xb + (size_t)jl * xp + cst[b]);
+      }
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        const bool valid = rr + v < n;
+        const T s = sv[v];
+        const T yy = valid ? yv[v] : T(0);
+        T r;
+        if (LINEAR) {
+          const T res = yy - s;
+          r = valid ? res * inv_s2 : T(0);
+          lpl += valid ? (double)(T(-0.5) * res * res * inv_s2) : 0.0;
+          ssq += valid ? res * res : T(0);
+        } else {
+          const T w0 = T(1) - yy;
+          const SoftplusParts<T> sp = softplus_parts(s);
+          const T l0 = w0 == T(0) ? T(0) : -(fmax(s, T(0)) + sp.l) * w0;
+          const T l1 = yy == T(0) ? T(0) : -(fmax(-s, T(0)) + sp.l) * yy;
+          lpl += valid ? (double)(l0 + l1) : 0.0;
+          r = valid ? yy - sp.sig : T(0);
+        }
+        rbuf[V * lane + v] = r;
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      int j0 = 0;
+      for (; j0 + 4 * B < rows; j0 += 4 * B) {
+        VT nv[B][NBB];
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+          const int jj = j0 + 4 * B + 4 * u + ph;
+          const int jl = jj < rows ? jj : rows - 1;
+#pragma unroll
+          for (int b = 0; b < NBB; ++b)
+            nv[u][b] = *(const VT*)(xb + (size_t)jl * xp + cst[b]);
+        }
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+          const int jj = j0 + 4 * u + ph;
+          const T rj = jj < rows ? rbuf[jj] : T(0);
+#pragma unroll
+          for (int b = 0; b < NBB; ++b) acc[b] += gv[u][b] * rj;
+        }
+#pragma unroll
+        for (int u = 0; u < B; ++u)
+#pragma unroll
+          for (int b = 0; b < NBB; ++b) gv[u][b] = nv[u][b];
+      }
+#pragma unroll
+      for (int u = 0; u < B; ++u) {
+        const int jj = j0 + 4 * u + ph;
+        const T rj = jj < rows ? rbuf[jj] : T(0);
+#pragma unroll
+        for (int b = 0; b < NBB; ++b) acc[b] += gv[u][b] * rj;
+      }
+      // the next chunk overwrites rbuf only after every lane has read it
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+  }
+
   // Bayesian GLM.  LINEAR=false: logistic regression over beta (dim = D_x).
   // LINEAR=true: [beta, sigma] with Gaussian likelihood (dim = D_x + 1).
   // s = X beta is computed with one row per lane (X^T reads coalesced);
   // X^T r is accumulated with one state element per lane (X reads
   // coalesced), r passing through LDS 64 rows at a time.
-  // X^T r contribution of one chunk: lane (phase ph = lane / 16, group
-  // grp = lane % 16) adds rows ph, ph + 4, ... of the chunk, coefficients
-  // b*16V + V*grp + [0, V) of block b, read as one 16-byte vector per row
-  // and block (one load instruction covers 4 rows x 16V coefficients).
-  // Rows past `rows` are clamped and weighted by zero; blocks past the row
-  // pitch are clamped to the row start (their accumulators are never read).
-  template <int NBB>
-  __device__ void glm_grad_chunk(const T* xb, int xp, int rows, const T* rbuf,
-                                 const int (&cst)[kGlmBlocks<T, K>],
-                                 typename Vec16<T>::type (&acc)[kGlmBlocks<T, K>]) {
-    using VT = typename Vec16<T>::type;
-    constexpr int B = NBB == 1 ? 8 : 4;  // rows of loads in flight per phase
-    const int ph = lane_id() >> 4;
-    for (int j0 = 0; j0 < rows; j0 += 4 * B) {
-      VT v[B][NBB];
-      T rj[B];
-#pragma unroll
-      for (int u = 0; u < B; ++u) {
-        const int jj = j0 + 4 * u + ph;
-        const int jl = jj < rows ? jj : rows - 1;
-        rj[u] = jj < rows ? rbuf[jj] : T(0);
-#pragma unroll
-        for (int b = 0; b < NBB; ++b)
-          v[u][b] = *(const VT*)(xb + (size_t)jl * xp + cst[b]);
-      }
-#pragma unroll
-      for (int u = 0; u < B; ++u)
-#pragma unroll
-        for (int b = 0; b < NBB; ++b) acc[b] += v[u][b] * rj[u];
-    }
-  }
-
   template <bool LINEAR>
   __device__ double glm(const T (&th)[K], T (&g)[K], double& lpu) {
     using VT = typename Vec16<T>::type;
@@ -439,8 +547,9 @@ struct Target {
       inv_s2 = T(1) / (sigma * sigma);
     }
     // X and X^T rows are 16-byte aligned (pitches a multiple of 4); any
-    // other layout gives NaN rather than an unaligned read
-    if (((x_pitch | xt_pitch) & 3) != 0 || x_pitch < dx || xt_pitch < n) {
+    // other layout, or an empty X, gives NaN rather than a stray read
+    if (((x_pitch | xt_pitch) & 3) != 0 || x_pitch < dx || xt_pitch < n ||
+        dx < 1 || n < 1) {
 #pragma unroll
       for (int k = 0; k < K; ++k) g[k] = T(NAN);
       lpu = (double)NAN;
@@ -470,71 +579,16 @@ struct Target {
     const int nbl = (dx + CPB - 1) / CPB;  // live blocks
     double lpl = 0.0;
     T ssq = T(0);
-    for (int n0 = split_w * 64 * V; n0 < n; n0 += split_n * 64 * V) {
-      // s for rows n0 + V*lane + [0, V): one 16-byte load of X^T per
-      // coefficient (rows past the padded end are clamped and masked)
-      const int rr = n0 + V * lane;
-      const T* col = xt + (rr < tp ? rr : tp - V);
-      VT s0 = VT{}, s1 = VT{};
-      int d = 0;
-      for (; d + 8 <= dx; d += 8) {
-        VT v[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = *(const VT*)(col + (size_t)(d + u) * tp);
-#pragma unroll
-        for (int u = 0; u < 8; u += 2) {
-          s0 += v[u] * beta[d + u];
-          s1 += v[u + 1] * beta[d + u + 1];
-        }
-      }
-      for (; d + 2 <= dx; d += 2) {
-        const VT v0 = *(const VT*)(col + (size_t)d * tp);
-        const VT v1 = *(const VT*)(col + (size_t)(d + 1) * tp);
-        s0 += v0 * beta[d];
-        s1 += v1 * beta[d + 1];
-      }
-      if (d < dx) s0 += *(const VT*)(col + (size_t)d * tp) * beta[d];
-      const VT sv = s0 + s1;
-#pragma unroll
-      for (int v = 0; v < V; ++v) {
-        const int row = rr + v;
-        const bool valid = row < n;
-        const T s = sv[v];
-        const T yv = valid ? y[row] : T(0);
-        T r;
-        if (LINEAR) {
-          const T res = yv - s;
-          r = valid ? res * inv_s2 : T(0);
-          lpl += valid ? (double)(T(-0.5) * res * res * inv_s2) : 0.0;
-          ssq += valid ? res * res : T(0);
-        } else {
-          const T w0 = T(1) - yv;
-          const SoftplusParts<T> sp = softplus_parts(s);
-          const T l0 = w0 == T(0) ? T(0) : -(fmax(s, T(0)) + sp.l) * w0;
-          const T l1 = yv == T(0) ? T(0) : -(fmax(-s, T(0)) + sp.l) * yv;
-          lpl += valid ? (double)(l0 + l1) : 0.0;
-          r = valid ? yv - sp.sig : T(0);
-        }
-        rbuf[V * lane + v] = r;
-      }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      const int rows = (n - n0) < 64 * V ? (n - n0) : 64 * V;
-      const T* xb = x + (size_t)n0 * xp;
-      if (nbl <= 1 || NB == 1) {
-        glm_grad_chunk<1>(xb, xp, rows, rbuf, cst, acc);
-      } else if (nbl == 2 || NB == 2) {
-        glm_grad_chunk<(NB >= 2 ? 2 : 1)>(xb, xp, rows, rbuf, cst, acc);
-      } else if (nbl == 3 || NB == 3) {
-        glm_grad_chunk<(NB >= 3 ? 3 : 1)>(xb, xp, rows, rbuf, cst, acc);
-      } else {
-        glm_grad_chunk<NB>(xb, xp, rows, rbuf, cst, acc);
-      }
-      // the next chunk overwrites rbuf only after every lane has read it
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (nbl <= 1 || NB == 1) {
+      glm_rows<1, LINEAR>(beta, rbuf, dx, xp, tp, inv_s2, cst, acc, lpl, ssq);
+    } else if (nbl == 2 || NB == 2) {
+      glm_rows<(NB >= 2 ? 2 : 1), LINEAR>(beta, rbuf, dx, xp, tp, inv_s2, cst,
+                                          acc, lpl, ssq);
+    } else if (nbl == 3 || NB == 3) {
+      glm_rows<(NB >= 3 ? 3 : 1), LINEAR>(beta, rbuf, dx, xp, tp, inv_s2, cst,
+                                          acc, lpl, ssq);
+    } else {
+      glm_rows<NB, LINEAR>(beta, rbuf, dx, xp, tp, inv_s2, cst, acc, lpl, ssq);
     }
     // sum the four row phases of each coefficient in a fixed order
 #pragma unroll

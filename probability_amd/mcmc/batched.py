@@ -24,7 +24,7 @@ from probability_amd import bijectors as bij_lib
 from probability_amd import random as samplers
 from probability_amd import targets as targets_lib
 
-__all__ = ['PosteriorBatch', 'sample_posteriors', 'BatchResults']
+__all__ = ['PosteriorBatch', 'sample_posteriors', 'sample_ragged', 'BatchResults']
 
 BatchResults = collections.namedtuple('BatchResults', [
     'draws', 'step_size', 'log_accept_ratio', 'leapfrogs_taken',
@@ -53,7 +53,7 @@ class PosteriorBatch:
       per lane, chosen from the largest dim of the launch; passing the same
       slot_dim to every batch a problem may run in makes it run the same
       code (bit-identical results) alone or in any mix.
-    waves_per_chain: GLMs only, waves sharing one chain's gradient (1-4;
+    waves_per_chain: GLMs only, waves sharing one chain's gradient (1-8;
       None: chosen from the largest design matrix).  Like slot_dim, equal
       values give bit-identical results alone or in any mix.
   """
@@ -158,119 +158,279 @@ def sample_posteriors(batch, num_results, num_burnin_steps=0,
   zeros).  step_size: scalar or one value per problem.  progress, if given,
   is called as progress(transitions_enqueued, total) after each launch.
   """
-  if kernel not in ('nuts', 'hmc'):
-    raise ValueError("kernel must be 'nuts' or 'hmc'")
-  _lib.require_cuda()
-  dev, dt = batch.device, batch.dtype
-  c, d, p = batch.n_chains, batch.dim_stride, batch.n_problems
-  if init_state is None:
-    x = torch.zeros((c, d), dtype=dt, device=dev)
-  else:
-    x = batch.bijector(torch.as_tensor(init_state, dtype=dt, device=dev)
-                       .contiguous(), 1)
-  tlp, grad = batch.eval(x)
-  step = torch.as_tensor(np.broadcast_to(np.asarray(step_size, np.float64),
-                                         (p,)).copy(), device=dev).to(dt)
-  step64 = step.to(torch.float64).clone()
-  err = torch.zeros(p, dtype=torch.float64, device=dev)
-  avg = torch.zeros_like(err)
-  lst = np.log(10.0) + torch.log(step64)
-  tgt = torch.as_tensor(np.broadcast_to(np.asarray(target_accept_prob,
-                                                   np.float64), (p,)).copy(),
-                        device=dev)
-  n_total = num_burnin_steps + num_results
-  rows = max(n_total, 1)
-  lar = torch.empty((rows, c), dtype=dt, device=dev)
-  lf = torch.zeros((rows, c), dtype=torch.int32, device=dev)
-  acc = torch.empty((rows, c), dtype=torch.uint8, device=dev)
-  div = torch.zeros((rows, c), dtype=torch.uint8, device=dev)
-  tr = torch.empty((num_results, c, d), dtype=dt, device=dev) if trace else None
-  base_seed = samplers.sanitize_seed(seed, salt='mcmc.sample_chain')
-  stream = _lib.stream_handle(dev)
-  fn = (_lib.lib().hipmc_nuts_run if kernel == 'nuts'
-        else _lib.lib().hipmc_hmc_run)
-  events = [] if (stats is not None and stats.get('time_kernels')) else None
+  run = _Run(batch, torch.cuda.current_stream(batch.device), num_results,
+             num_burnin_steps, num_adaptation_steps, init_state, step_size,
+             target_accept_prob, max_tree_depth, max_energy_diff, kernel,
+             num_leapfrog_steps, seed, trace, stats, max_iters_per_launch)
+  while not run.done():
+    run.advance()
+    if progress is not None:
+      progress(run.t, run.n_total)
+  return run.finish()
 
-  def launch(t0, n):
+
+def sample_ragged(specs, num_chains, num_results, bijectors=None, keys=None,
+                  slot_dim=None, progress=None, stats=None, **kwargs):
+  """sample_posteriors for problems of very different sizes: the GLMs are
+  split into classes by the waves that share one of their chains (from each
+  problem's own N x D), every class is its own batch on its own stream, and
+  the classes advance concurrently, each at its own pace (small problems do
+  not wait for the largest ones' adaptation steps).  A problem's draws
+  depend only on its own size class, key and chain count, so they equal its
+  draws alone.  Returns BatchResults in the order of `specs`.
+  """
+  specs = [targets_lib.as_target(s) for s in specs]
+  p = len(specs)
+  chains = ([int(num_chains)] * p if np.isscalar(num_chains)
+            else [int(c) for c in num_chains])
+  bijs = bijectors or [None] * p
+  keys = list(range(p)) if keys is None else [int(k) for k in keys]
+  glm = _family(specs[0].kind) == 1
+  cls = [_lib.glm_waves_per_chain(s.n_obs * s.dim) if glm else 1
+         for s in specs]
+  dev = specs[0].device
+  caller = torch.cuda.current_stream(dev)
+  runs = []
+  for w in sorted(set(cls), reverse=True):
+    idx = [i for i in range(p) if cls[i] == w]
+    if glm:
+      idx = _xcd_order(idx, [specs[i].n_obs * specs[i].dim for i in idx])
+    batch = PosteriorBatch([specs[i] for i in idx], [chains[i] for i in idx],
+                           [bijs[i] for i in idx], [keys[i] for i in idx],
+                           slot_dim=slot_dim, waves_per_chain=w)
+    stream = torch.cuda.Stream(device=dev)
+    stream.wait_stream(caller)
+    sub = {} if stats is not None else None
+    if sub is not None and stats.get('time_kernels'):
+      sub['time_kernels'] = True
+    runs.append((idx, sub, _Run(batch, stream, num_results, stats=sub,
+                                **kwargs)))
+  total = sum(r.n_total for _, _, r in runs)
+  while not all(r.done() for _, _, r in runs):
+    for _, _, r in runs:
+      if not r.done():
+        r.advance()
+    if progress is not None:
+      progress(sum(r.t for _, _, r in runs), total)
+  parts = [(idx, sub, r.finish(consumer=caller)) for idx, sub, r in runs]
+  for _, _, r in runs:
+    caller.wait_stream(r.stream)
+  out = {f: [None] * p for f in BatchResults._fields}
+  step = torch.empty(p, dtype=specs[0].dtype, device=dev)
+  for idx, _, res in parts:
+    for f in BatchResults._fields:
+      if f == 'step_size':
+        step[torch.as_tensor(idx, device=dev)] = res.step_size
+      elif getattr(res, f) is not None:
+        for j, i in enumerate(idx):
+          out[f][i] = getattr(res, f)[j]
+  if stats is not None:
+    stats['leapfrogs_total'] = sum(sub['leapfrogs_total'] for _, sub, _ in parts)
+    lpp = np.zeros(p)
+    for idx, sub, _ in parts:
+      lpp[idx] = sub['leapfrogs_per_problem']
+    stats['leapfrogs_per_problem'] = lpp
+    stats['classes'] = [(len(idx), r.batch.waves_per_chain)
+                        for idx, _, r in runs]
+    stats['class_stats'] = [{k: sub[k] for k in (
+        'cost_total', 'cost_adaptation', 'cost_critical',
+        'cost_critical_adaptation', 'leapfrogs_total', 'launches')
+                             if k in sub} for _, sub, _ in parts]
+    if stats.get('time_kernels'):
+      stats['launches'] = [l for _, sub, _ in parts for l in sub['launches']]
+  out['step_size'] = step
+  if not kwargs.get('trace', True):
+    out['draws'] = None
+  return BatchResults(**out)
+
+
+def _xcd_order(idx, cost, n_xcd=8):
+  """Launch order for problems of unequal cost.  The kernels give XCD x the
+  x-th contiguous eighth of the launch's blocks and each XCD starts its
+  blocks in order, so: deal the problems, most expensive first, to the
+  eight ranges in snake order (equal counts, similar total cost, each
+  problem's chains on one XCD and its L2), and run each range from its
+  most expensive problem down (long trajectories start first and small
+  problems fill in around them)."""
+  order = sorted(range(len(idx)), key=lambda j: -cost[j])
+  bins = [[] for _ in range(n_xcd)]
+  for r, j in enumerate(order):
+    k = r % (2 * n_xcd)
+    bins[k if k < n_xcd else 2 * n_xcd - 1 - k].append(idx[j])
+  return [i for b in bins for i in b]
+
+
+class _Run:
+  """One batch's sampling loop, advanced one launch at a time on `stream`."""
+
+  def __init__(self, batch, stream, num_results, num_burnin_steps=0,
+               num_adaptation_steps=0, init_state=None, step_size=0.1,
+               target_accept_prob=0.75, max_tree_depth=10,
+               max_energy_diff=1000.0, kernel='nuts', num_leapfrog_steps=10,
+               seed=None, trace=True, stats=None, max_iters_per_launch=4096):
+    if kernel not in ('nuts', 'hmc'):
+      raise ValueError("kernel must be 'nuts' or 'hmc'")
+    _lib.require_cuda()
+    self.batch, self.stream = batch, stream
+    self.kernel = kernel
+    self.num_results, self.num_burnin = num_results, num_burnin_steps
+    self.num_adapt = num_adaptation_steps
+    self.max_tree_depth, self.max_energy_diff = max_tree_depth, max_energy_diff
+    self.num_leapfrog_steps = num_leapfrog_steps
+    self.stats = stats
+    self.max_iters = max_iters_per_launch
+    dev, dt = batch.device, batch.dtype
+    c, d, p = batch.n_chains, batch.dim_stride, batch.n_problems
+    with torch.cuda.stream(stream):
+      if init_state is None:
+        self.x = torch.zeros((c, d), dtype=dt, device=dev)
+      else:
+        self.x = batch.bijector(torch.as_tensor(init_state, dtype=dt,
+                                                device=dev).contiguous(), 1)
+      self.tlp, self.grad = batch.eval(self.x)
+      self.step = torch.as_tensor(np.broadcast_to(
+          np.asarray(step_size, np.float64), (p,)).copy(),
+                                  device=dev).to(dt)
+      self.step64 = self.step.to(torch.float64).clone()
+      self.err = torch.zeros(p, dtype=torch.float64, device=dev)
+      self.avg = torch.zeros_like(self.err)
+      self.lst = np.log(10.0) + torch.log(self.step64)
+      self.tgt = torch.as_tensor(np.broadcast_to(
+          np.asarray(target_accept_prob, np.float64), (p,)).copy(), device=dev)
+      self.n_total = num_burnin_steps + num_results
+      rows = max(self.n_total, 1)
+      self.lar = torch.empty((rows, c), dtype=dt, device=dev)
+      self.lf = torch.zeros((rows, c), dtype=torch.int32, device=dev)
+      self.acc = torch.empty((rows, c), dtype=torch.uint8, device=dev)
+      self.div = torch.zeros((rows, c), dtype=torch.uint8, device=dev)
+      self.tr = (torch.empty((num_results, c, d), dtype=dt, device=dev)
+                 if trace else None)
+    self.base_seed = samplers.sanitize_seed(seed, salt='mcmc.sample_chain')
+    self.fn = (_lib.lib().hipmc_nuts_run if kernel == 'nuts'
+               else _lib.lib().hipmc_hmc_run)
+    self.events = ([] if (stats is not None and stats.get('time_kernels'))
+                   else None)
+    self.t = 0
+
+  def done(self):
+    return self.t >= self.n_total
+
+  def advance(self):
+    """Enqueues the next launch (and its dual-averaging update)."""
+    with torch.cuda.stream(self.stream):
+      t = self.t
+      n = 1 if t < self.num_adapt else min(self.n_total - t, self.max_iters)
+      self._launch(t, n)
+      if self.num_adapt > 0:
+        self._da(t, n)
+      self.t = t + n
+
+  def _launch(self, t0, n):
+    b, dt = self.batch, self.batch.dtype
     a = _lib.RunArgs()
     a.dtype = _lib.dtype_code(dt)
-    a.n_chains = c
-    a.dim_stride = d
-    a.max_dim = batch.max_dim
+    a.n_chains = b.n_chains
+    a.dim_stride = b.dim_stride
+    a.max_dim = b.max_dim
     a.n_iters = n
     a.fold_seed = 1
     a.iter0 = t0
-    a.seed[0], a.seed[1] = int(base_seed[0]), int(base_seed[1])
-    a.num_leapfrog_steps = int(num_leapfrog_steps)
-    a.max_tree_depth = int(max_tree_depth)
+    a.seed[0], a.seed[1] = int(self.base_seed[0]), int(self.base_seed[1])
+    a.num_leapfrog_steps = int(self.num_leapfrog_steps)
+    a.max_tree_depth = int(self.max_tree_depth)
     a.unrolled_leapfrog_steps = 1
-    a.target_kind = batch.kind
-    a.waves_per_chain = batch.waves_per_chain
-    a.max_energy_diff = float(max_energy_diff)
-    a.problems = batch.table.data_ptr()
-    a.n_problems = p
-    a.chain_problem = batch.chain_problem.data_ptr()
-    a.chain_local = batch.chain_local.data_ptr()
-    a.step_size = step.data_ptr()
-    a.chain_step = batch.chain_problem.data_ptr()
-    a.state, a.tlp, a.grad = x.data_ptr(), tlp.data_ptr(), grad.data_ptr()
+    a.target_kind = b.kind
+    a.waves_per_chain = b.waves_per_chain
+    a.max_energy_diff = float(self.max_energy_diff)
+    a.problems = b.table.data_ptr()
+    a.n_problems = b.n_problems
+    a.chain_problem = b.chain_problem.data_ptr()
+    a.chain_local = b.chain_local.data_ptr()
+    a.step_size = self.step.data_ptr()
+    a.chain_step = b.chain_problem.data_ptr()
+    a.state, a.tlp = self.x.data_ptr(), self.tlp.data_ptr()
+    a.grad = self.grad.data_ptr()
     a.out_row0 = t0
-    a.out_log_accept_ratio = lar.data_ptr()
-    a.out_is_accepted = acc.data_ptr()
-    if kernel == 'nuts':
-      a.out_leapfrogs = lf.data_ptr()
-      a.out_has_divergence = div.data_ptr()
-    if tr is not None:
-      a.trace_state = tr.data_ptr()
-      a.trace_start = num_burnin_steps
+    a.out_log_accept_ratio = self.lar.data_ptr()
+    a.out_is_accepted = self.acc.data_ptr()
+    if self.kernel == 'nuts':
+      a.out_leapfrogs = self.lf.data_ptr()
+      a.out_has_divergence = self.div.data_ptr()
+    if self.tr is not None:
+      a.trace_state = self.tr.data_ptr()
+      a.trace_start = self.num_burnin
       a.trace_every = 1
-      a.n_trace = num_results
-    if events is not None:
+      a.n_trace = self.num_results
+    if self.events is not None:
       e0 = torch.cuda.Event(enable_timing=True)
       e1 = torch.cuda.Event(enable_timing=True)
       e0.record()
-    _lib.check(fn(a, stream))
-    if events is not None:
+    _lib.check(self.fn(a, _lib.stream_handle(b.device)))
+    if self.events is not None:
       e1.record()
-      events.append((e0, e1, t0, n))
+      self.events.append((e0, e1, t0, n))
 
-  def da(t0, n):
+  def _da(self, t0, n):
+    b = self.batch
     _lib.check(_lib.lib().hipmc_da_update(
-        _lib.dtype_code(dt), p, None, None, _lib.ptr(batch.group_start),
-        _lib.ptr(lar[t0]), n, c, _lib.ptr(err), _lib.ptr(avg), _lib.ptr(lst),
-        _lib.ptr(tgt), t0, num_adaptation_steps, 0.05, 10.0, 0.75,
-        _lib.ptr(step), _lib.ptr(step64), stream))
+        _lib.dtype_code(b.dtype), b.n_problems, None, None,
+        _lib.ptr(b.group_start), _lib.ptr(self.lar[t0]), n, b.n_chains,
+        _lib.ptr(self.err), _lib.ptr(self.avg), _lib.ptr(self.lst),
+        _lib.ptr(self.tgt), t0, self.num_adapt, 0.05, 10.0, 0.75,
+        _lib.ptr(self.step), _lib.ptr(self.step64),
+        _lib.stream_handle(b.device)))
 
-  t = 0
-  while t < n_total:
-    n = 1 if t < num_adaptation_steps else min(n_total - t,
-                                                max_iters_per_launch)
-    launch(t, n)
-    if num_adaptation_steps > 0:
-      da(t, n)
-    t += n
-    if progress is not None:
-      progress(t, n_total)
-  if stats is not None:
-    stats['leapfrogs_per_step'] = lf[:n_total].sum(1)
-    stats['leapfrogs_total'] = int(lf[:n_total].sum())
-    stats['leapfrogs_per_problem'] = torch.zeros(
-        p, dtype=torch.float64, device=dev).index_add_(
-            0, batch.chain_problem.long(), lf[:n_total].sum(0).double()
-        ).cpu().numpy()
-    if events is not None:
-      torch.cuda.synchronize(dev)
-      stats['launches'] = [(e0.elapsed_time(e1), t0, n)
-                           for e0, e1, t0, n in events]
-  draws = None
-  if tr is not None:
-    con = batch.bijector(tr, 0, chain_period=c) if tr.numel() else tr
-    draws = batch.split(con)
-  rec = slice(num_burnin_steps, n_total)
-  return BatchResults(
-      draws=draws, step_size=step.clone(),
-      log_accept_ratio=batch.split_chains(lar[rec]),
-      leapfrogs_taken=batch.split_chains(lf[rec]),
-      is_accepted=batch.split_chains(acc[rec].bool()),
-      has_divergence=batch.split_chains(div[rec].bool()),
-      final_state=batch.split(batch.bijector(x, 0)))
+  def finish(self, consumer=None):
+    """Per-problem results (and stats) once every launch is enqueued.  With
+    `consumer`, the result buffers are marked as used on that stream too, so
+    the allocator does not hand them out again before its work is done."""
+    b, n_total, dev = self.batch, self.n_total, self.batch.device
+    with torch.cuda.stream(self.stream):
+      lf = self.lf
+      if self.stats is not None:
+        self.stats['leapfrogs_per_step'] = lf[:n_total].sum(1)
+        self.stats['leapfrogs_total'] = int(lf[:n_total].sum())
+        self.stats['leapfrogs_per_problem'] = torch.zeros(
+            b.n_problems, dtype=torch.float64, device=dev).index_add_(
+                0, b.chain_problem.long(), lf[:n_total].sum(0).double()
+            ).cpu().numpy()
+        # per-chain cost of one leapfrog (design-matrix entries for GLMs,
+        # state dimension otherwise): the busiest chain of each adaptation
+        # step, and of each sampling launch, bounds that launch's time
+        w = torch.as_tensor([float(getattr(s, 'n_obs', 1) * s.dim)
+                             for s in b.specs], dtype=torch.float64,
+                            device=dev)[b.chain_problem.long()]
+        lfw = lf[:n_total].double() * w
+        na = min(self.num_adapt, n_total)
+        crit = float(lfw[:na].max(1).values.sum()) if na else 0.0
+        for t0 in range(na, n_total, self.max_iters):
+          crit += float(lfw[t0:t0 + self.max_iters].sum(0).max())
+        self.stats['cost_total'] = float(lfw.sum())
+        self.stats['cost_adaptation'] = float(lfw[:na].sum())
+        self.stats['cost_critical'] = crit
+        self.stats['cost_critical_adaptation'] = (
+            float(lfw[:na].max(1).values.sum()) if na else 0.0)
+        if self.events is not None:
+          self.stream.synchronize()
+          self.stats['launches'] = [(e0.elapsed_time(e1), t0, n)
+                                    for e0, e1, t0, n in self.events]
+      draws = con = None
+      if self.tr is not None:
+        con = (b.bijector(self.tr, 0, chain_period=b.n_chains)
+               if self.tr.numel() else self.tr)
+        draws = b.split(con)
+      rec = slice(self.num_burnin, n_total)
+      step = self.step.clone()
+      acc, div = self.acc[rec].bool(), self.div[rec].bool()
+      final = b.bijector(self.x, 0)
+      if consumer is not None:
+        for t in (con, step, self.lar, lf, acc, div, final):
+          if t is not None:
+            t.record_stream(consumer)
+      return BatchResults(
+          draws=draws, step_size=step,
+          log_accept_ratio=b.split_chains(self.lar[rec]),
+          leapfrogs_taken=b.split_chains(lf[rec]),
+          is_accepted=b.split_chains(acc),
+          has_divergence=b.split_chains(div),
+          final_state=b.split(final))

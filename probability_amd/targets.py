@@ -230,6 +230,8 @@ class _GlmData:
   """X stored row-major and transposed, each with padded 16-byte rows."""
 
   def _set_design(self, x):
+    if x.dim() != 2 or x.shape[0] < 1 or x.shape[1] < 1:
+      raise ValueError('x must be a non-empty [N, D] matrix')
     self._xbuf, self.x = _padded_rows(x)
     self._xtbuf, self.xt = _padded_rows(x.t())
 

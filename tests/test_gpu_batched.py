@@ -83,3 +83,44 @@ def test_batched_matches_oracle(dtype, ndt, tol):
     np.testing.assert_array_equal(res.leapfrogs_taken[k].cpu().numpy(), lf)
     np.testing.assert_allclose(res.draws[k].cpu().numpy(), ostates, rtol=tol,
                                atol=tol)
+
+
+def _ragged_problems(dtype):
+  r = np.random.default_rng(11)
+  specs, bij = [], []
+  # N x D spans the four wave-count classes (<4096, <16384, <262144,
+  # >= 262144).
+  for nobs, d in ((50, 7), (600, 40), (120, 12), (300, 20), (40, 3),
+                  (2700, 100)):
+    x = r.normal(size=(nobs, d)).astype(np.float32)
+    beta = r.normal(size=d) * 0.3
+    y = (r.uniform(size=nobs) < 1 / (1 + np.exp(-(x @ beta)))).astype(
+        np.float32)
+    specs.append(ptg.LogisticRegression(x, y, dtype=dtype))
+    bij.append(None)
+  return specs, bij
+
+
+@pytest.mark.parametrize('dtype', [torch.float32, torch.float64])
+def test_sample_ragged_equals_each_problem_alone(dtype):
+  from probability_amd import _lib
+  specs, bij = _ragged_problems(dtype)
+  chains = [4, 6, 5, 3, 7, 3]
+  kw = dict(num_results=4, num_burnin_steps=5, num_adaptation_steps=5,
+            step_size=0.05, target_accept_prob=0.8, seed=9)
+  stats = {}
+  mix = batched.sample_ragged(specs, chains, bijectors=bij, slot_dim=64,
+                              stats=stats, **kw)
+  assert sorted(w for _, w in stats['classes']) == [1, 2, 4, 8]
+  for k, s in enumerate(specs):
+    w = _lib.glm_waves_per_chain(s.n_obs * s.dim)
+    alone = batched.sample_posteriors(
+        batched.PosteriorBatch([s], [chains[k]], [bij[k]], keys=[k],
+                               slot_dim=64, waves_per_chain=w), **kw)
+    np.testing.assert_array_equal(mix.draws[k].cpu().numpy(),
+                                  alone.draws[0].cpu().numpy())
+    np.testing.assert_array_equal(mix.leapfrogs_taken[k].cpu().numpy(),
+                                  alone.leapfrogs_taken[0].cpu().numpy())
+    np.testing.assert_array_equal(mix.final_state[k].cpu().numpy(),
+                                  alone.final_state[0].cpu().numpy())
+    assert float(mix.step_size[k]) == float(alone.step_size[0])

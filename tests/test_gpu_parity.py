@@ -49,6 +49,14 @@ def _case(kind, dtype_np, seed=0, c=64):
     mk = lambda dt: ptg.LogisticRegression(x, y, 1.0, dtype=dt)
     q = cast(r.normal(size=(c, 9)) * 0.1)
     return o, mk, q, None
+  if kind == 'logistic_wide':
+    # 2700 x 100: eight waves share each chain's gradient
+    x = cast(r.normal(size=(2700, 100)) * 0.1)
+    y = (r.uniform(size=2700) < 0.5).astype(np.float64)
+    o = otg.LogisticRegression(x, y, 1.0)
+    mk = lambda dt: ptg.LogisticRegression(x, y, 1.0, dtype=dt)
+    q = cast(r.normal(size=(c, 100)) * 0.03)
+    return o, mk, q, None
   if kind == 'linear_split':
     x = cast(r.normal(size=(1500, 11)))
     y = cast(x @ r.normal(size=11) * 0.3 + 0.5 * r.normal(size=1500))
@@ -150,7 +158,8 @@ def test_draws_match_oracle(tdt, ndt, tol):
 # ---------------------------------------------------------------- targets
 @pytest.mark.parametrize('tdt,ndt,tol', DTYPES)
 @pytest.mark.parametrize('kind', ['logistic', 'linear', 'hier', 'diag',
-                                  'funnel', 'logistic_split', 'linear_split'])
+                                  'funnel', 'logistic_split', 'linear_split',
+                                  'logistic_wide'])
 def test_target_eval_matches_oracle(kind, tdt, ndt, tol):
   o, mk, q, codes = _case(kind, ndt)
   spec = mk(tdt)
@@ -176,13 +185,14 @@ def test_target_callable_like_tfp(tdt, ndt, tol):
 
 # ---------------------------------------------------------------- HMC
 @pytest.mark.parametrize('tdt,ndt,tol', DTYPES)
-@pytest.mark.parametrize('kind', ['logistic', 'hier', 'logistic_split'])
+@pytest.mark.parametrize('kind', ['logistic', 'hier', 'logistic_split',
+                                  'logistic_wide'])
 def test_hmc_one_step_matches_oracle(kind, tdt, ndt, tol):
   o, mk, q, codes = _case(kind, ndt, seed=3)
   spec = mk(tdt)
   ot = _otarget(o, codes)
   eps, nl = {'logistic': (0.05, 10), 'hier': (0.2, 6),
-             'logistic_split': (0.05, 8)}[kind]
+             'logistic_split': (0.05, 8), 'logistic_wide': (0.02, 8)}[kind]
   kern = pmcmc.HamiltonianMonteCarlo(
       spec if codes is None else ptg.TransformedTarget(spec, _bijs(spec, codes)),
       step_size=eps, num_leapfrog_steps=nl)
@@ -237,7 +247,7 @@ def _nuts_one(kind, tdt, ndt, seed=0, c=64, eps=None, max_depth=10):
   ot = _otarget(o, codes)
   eps = eps or {'logistic': 0.08, 'linear': 0.05, 'hier': 0.3, 'diag': 0.5,
                 'funnel': 0.9, 'logistic_split': 0.04,
-                'linear_split': 0.02}[kind]
+                'linear_split': 0.02, 'logistic_wide': 0.02}[kind]
   tspec = spec if codes is None else ptg.TransformedTarget(spec,
                                                            _bijs(spec, codes))
   kern = pmcmc.NoUTurnSampler(tspec, step_size=np.asarray(eps, ndt),
@@ -256,7 +266,8 @@ def _nuts_one(kind, tdt, ndt, seed=0, c=64, eps=None, max_depth=10):
 
 @pytest.mark.parametrize('tdt,ndt,tol', DTYPES)
 @pytest.mark.parametrize('kind', ['hier', 'logistic', 'linear', 'diag',
-                                  'funnel', 'logistic_split', 'linear_split'])
+                                  'funnel', 'logistic_split', 'linear_split',
+                                  'logistic_wide'])
 def test_nuts_one_step_matches_oracle(kind, tdt, ndt, tol):
   for seed in range(3):
     spec, nxt, res, oq, ores = _nuts_one(kind, tdt, ndt, seed=seed)
