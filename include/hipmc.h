@@ -79,7 +79,10 @@ extern "C" {
  * layout is fixed (offsets asserted in the library).  Data pointers are in
  * the kernel dtype.
  *   LOGISTIC / LINEAR: x = X [n_obs, dim_x] row-major (row pitch x_pitch),
- *                      xt = X^T [dim_x, n_obs] (row pitch xt_pitch),
+ *                      xt = X^T [dim_x, n_obs] (row pitch xt_pitch) or
+ *                      NULL: read only when dim_x <= 64 (F32) / 32 (F64),
+ *                      for the logits; otherwise logits and gradient come
+ *                      from one pass over the rows of x,
  *                      y [n_obs]; param[0] = prior scale of beta,
  *                      param[1] = HalfNormal scale of sigma (LINEAR).
  *   HIER_NORMAL:       y [J], aux = sigma [J], n_obs = J;
@@ -105,9 +108,10 @@ typedef struct hipmc_problem {
   int32_t bij_len[HIPMC_MAX_BIJ_RANGES];
   int32_t bij_code[HIPMC_MAX_BIJ_RANGES];
   uint32_t key; /* problem word of the Philox counter (stable id) */
-  /* LOGISTIC / LINEAR: row pitches (elements) of x (>= dim_x) and xt
-   * (>= n_obs), multiples of 4 with 16-byte aligned x / xt: rows are read
-   * 16 bytes per lane (padding is read and ignored).  Other kinds: unused. */
+  /* LOGISTIC / LINEAR: row pitches (elements) of x (>= dim_x) and of a
+   * non-NULL xt (>= n_obs), multiples of 4 with x / xt 16-byte aligned:
+   * rows are read 16 bytes per lane (padding is read and ignored).  An xt
+   * whose pitch breaks these rules is not used.  Other kinds: unused. */
   int32_t x_pitch;
   int32_t xt_pitch;
 } hipmc_problem;

@@ -148,6 +148,32 @@ __device__ inline A row_sum(A v) {
   v = dpp_add<kDppHalfMirror>(v);
   return dpp_add<kDppMirror>(v);
 }
+// v of lane i of this lane's quad (DPP quad_perm [i, i, i, i]).
+template <int I>
+__device__ inline float quad_bcast(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v),
+                                                    I * 0x55, 0xF, 0xF, false));
+}
+template <int I>
+__device__ inline double quad_bcast(double v) {
+  const int64_t b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, I * 0x55,
+                                             0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32),
+                                             I * 0x55, 0xF, 0xF, false);
+  return __longlong_as_double(((int64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+// The same for an index that is a constant after unrolling.
+template <typename A>
+__device__ inline A quad_bcast(A v, int i) {
+  switch (i & 3) {
+    case 0: return quad_bcast<0>(v);
+    case 1: return quad_bcast<1>(v);
+    case 2: return quad_bcast<2>(v);
+    default: return quad_bcast<3>(v);
+  }
+}
+
 // Combines row sums r0..r3: rows 1 and 3 take r1 + r0 and r3 + r2, then row 3
 // takes (r3 + r2) + (r1 + r0), read from lane 63.  The result equals
 // (r0 + r1) + (r2 + r3) bit for bit (commutative adds).

@@ -56,13 +56,13 @@ __host__ __device__ constexpr size_t glm_red_bytes(int w) {
 }
 
 // Per-wave LDS scratch of a target evaluation: GLMs need beta [K*64], the
-// residuals of a chunk [64 V] and the gradient partials [64 NB V]; the
-// other families need [K*64 + 64].
+// residuals of a chunk [64 V] (narrow designs) and the gradient partials
+// [64 NB V]; the other families need [K*64 + 64].
 template <typename T, int K, int FAM>
 __host__ __device__ constexpr size_t target_scratch_elems() {
   return FAM == kFamilyGlm ? (size_t)K * 64 + 64 * Vec16<T>::n +
-                        64 * kGlmBlocks<T, K> * Vec16<T>::n
-                  : (size_t)K * 64 + 64;
+                                 64 * kGlmBlocks<T, K> * Vec16<T>::n
+                           : (size_t)K * 64 + 64;
 }
 
 // Bit-mask blends: v[s] for a wave-uniform slot index s, and
@@ -385,8 +385,9 @@ struct Target {
     return lpl;
   }
 
-  // The row loop of one GLM evaluation; this wave handles chunks of 64V
-  // rows n0 = split_w * 64V, advancing by split_n * 64V.
+  // The row loop of a narrow GLM (one coefficient block, X^T given): two
+  // passes per chunk.  This wave handles chunks of 64V rows
+  // n0 = split_w * 64V, advancing by split_n * 64V.
   //  s-pass: lane owns rows n0 + V*lane + [0, V) and reads one 16-byte
   //   vector of X^T per coefficient, 8 coefficients per batch; the next
   //   batch is issued before the current one is used, and coefficients past
@@ -525,11 +526,137 @@ struct Target {
     }
   }
 
+  // The row loop of a wider GLM evaluation: one pass over X.  This wave takes
+  // chunks of 64V rows n0 = split_w * 64V, advancing by split_n * 64V, in
+  // batches of 4B rows.  Lane (phase ph = lane / 16, group grp = lane % 16)
+  // holds rows r0 + 4u + ph (u < B) of a batch, coefficients
+  // b*16V + V*grp + [0, V) of block b, as one 16-byte vector per row and
+  // block (one load instruction covers 4 rows x 16V coefficients); the next
+  // batch is loaded while the current one is used.
+  //  logits: the lane's partial dot products with beta, summed over the 16
+  //   lanes of its DPP row (each of them gets the total);
+  //  residuals: lane evaluates rows u = (lane & 3) + 4k (k < B/4; for B = 2,
+  //   u = lane & 1) and the quad broadcasts them to its other lanes; only
+  //   lanes grp < min(B, 4) add the rows' log-likelihood terms;
+  //  gradient: acc[b] += x * r over the lane's coefficients.
+  // Rows past n read row n - 1 and get residual zero; coefficients past dx
+  // meet beta = 0; blocks past the row pitch are clamped to the row start
+  // (their accumulators are never read).
+  template <int NBB, bool LINEAR>
+  __device__ void glm_pass(const T* beta, int dx, int xp, T inv_s2,
+                           const int (&cst)[kGlmBlocks<T, K>],
+                           typename Vec16<T>::type (&acc)[kGlmBlocks<T, K>],
+                           double& lpl, T& ssq) {
+    using VT = typename Vec16<T>::type;
+    constexpr int V = Vec16<T>::n;
+    constexpr int CPB = 16 * V;
+    constexpr int B = NBB == 1 ? 8 : (NBB == 2 ? 4 : 2);
+    constexpr int RPL = B > 4 ? B / 4 : 1;  // rows evaluated per lane
+    constexpr int RB = 4 * B;               // rows per batch
+    constexpr int CH = 64 * V;              // rows per chunk
+    constexpr int NQ = CH / RB;             // batches per chunk
+    const int lane = lane_id();
+    const int ph = lane >> 4, grp = lane & 15;
+    const int n = n_obs;
+    const bool counts = grp < (B < 4 ? B : 4);
+    VT bs[NBB];
+#pragma unroll
+    for (int b = 0; b < NBB; ++b)
+#pragma unroll
+      for (int v = 0; v < V; ++v) {
+        const int c = b * CPB + V * grp + v;
+        bs[b][v] = c < dx ? beta[c] : T(0);
+      }
+    const int q0 = split_w * CH, qs = split_n * CH;
+    auto row0 = [&](int q) { return q0 + (q / NQ) * qs + (q % NQ) * RB; };
+    auto own = [&](int k) { return B >= 4 ? (lane & 3) + 4 * k : (lane & 1); };
+    auto load = [&](int r0, VT (&xv)[B][NBB], T (&yv)[RPL]) {
+#pragma unroll
+      for (int u = 0; u < B; ++u) {
+        const int j = r0 + 4 * u + ph;
+        const int jl = j < n ? j : n - 1;
+#pragma unroll
+        for (int b = 0; b < NBB; ++b)
+          xv[u][b] = *(const VT*)(x + (size_t)jl * xp + cst[b]);
+      }
+#pragma unroll
+      for (int k = 0; k < RPL; ++k) {
+        const int j = r0 + 4 * own(k) + ph;
+        yv[k] = y[j < n ? j : n - 1];
+      }
+    };
+    int r0 = row0(0);
+    if (r0 >= n) return;
+    VT xv[B][NBB];
+    T yv[RPL];
+    load(r0, xv, yv);
+    for (int q = 1;; ++q) {
+      const int r1 = row0(q);
+      const bool more = r1 < n;
+      VT nx[B][NBB];
+      T ny[RPL];
+      if (more) load(r1, nx, ny);
+      T sr[B];
+#pragma unroll
+      for (int u = 0; u < B; ++u) {
+        T a = T(0);
+#pragma unroll
+        for (int b = 0; b < NBB; ++b)
+#pragma unroll
+          for (int v = 0; v < V; ++v) a += xv[u][b][v] * bs[b][v];
+        sr[u] = row_sum(a);
+      }
+      T rr[RPL];
+#pragma unroll
+      for (int k = 0; k < RPL; ++k) {
+        T sq[B < 4 ? B : 4];
+#pragma unroll
+        for (int i = 0; i < (B < 4 ? B : 4); ++i) sq[i] = sr[(B < 4 ? 0 : 4 * k) + i];
+        const T s = pick(sq, B < 4 ? (lane & 1) : (lane & 3));
+        const bool valid = r0 + 4 * own(k) + ph < n;
+        const bool add = valid && counts;
+        T r;
+        if (LINEAR) {
+          const T res = yv[k] - s;
+          r = res * inv_s2;
+          lpl += add ? (double)(T(-0.5) * res * res * inv_s2) : 0.0;
+          ssq += add ? res * res : T(0);
+        } else {
+          const T yy = yv[k];
+          const T w0 = T(1) - yy;
+          const SoftplusParts<T> sp = softplus_parts(s);
+          const T l0 = w0 == T(0) ? T(0) : -(fmax(s, T(0)) + sp.l) * w0;
+          const T l1 = yy == T(0) ? T(0) : -(fmax(-s, T(0)) + sp.l) * yy;
+          lpl += add ? (double)(l0 + l1) : 0.0;
+          r = yy - sp.sig;
+        }
+        rr[k] = valid ? r : T(0);
+      }
+      T rj[B];
+#pragma unroll
+      for (int u = 0; u < B; ++u) rj[u] = quad_bcast(rr[u >> 2], u & 3);
+#pragma unroll
+      for (int u = 0; u < B; ++u)
+#pragma unroll
+        for (int b = 0; b < NBB; ++b) acc[b] += xv[u][b] * rj[u];
+      if (!more) break;
+      r0 = r1;
+#pragma unroll
+      for (int u = 0; u < B; ++u)
+#pragma unroll
+        for (int b = 0; b < NBB; ++b) xv[u][b] = nx[u][b];
+#pragma unroll
+      for (int k = 0; k < RPL; ++k) yv[k] = ny[k];
+    }
+  }
+
   // Bayesian GLM.  LINEAR=false: logistic regression over beta (dim = D_x).
   // LINEAR=true: [beta, sigma] with Gaussian likelihood (dim = D_x + 1).
-  // s = X beta is computed with one row per lane (X^T reads coalesced);
-  // X^T r is accumulated with one state element per lane (X reads
-  // coalesced), r passing through LDS 64 rows at a time.
+  // The logits X beta and the gradient X^T r come from one pass over the
+  // rows of X (glm_pass), or for a design of one coefficient block whose
+  // X^T is given, from a pass over X^T and one over X (glm_rows); the four
+  // row phases and, for a shared chain, the waves' partial sums are then
+  // added in a fixed order.
   template <bool LINEAR>
   __device__ double glm(const T (&th)[K], T (&g)[K], double& lpu) {
     using VT = typename Vec16<T>::type;
@@ -546,18 +673,17 @@ struct Target {
         if (k == (dx >> 6)) sigma = readlane(th[k], dx & 63);
       inv_s2 = T(1) / (sigma * sigma);
     }
-    // X and X^T rows are 16-byte aligned (pitches a multiple of 4); any
-    // other layout, or an empty X, gives NaN rather than a stray read
-    if (((x_pitch | xt_pitch) & 3) != 0 || x_pitch < dx || xt_pitch < n ||
-        dx < 1 || n < 1) {
+    // X rows are 16-byte aligned (pitch a multiple of 4); any other layout,
+    // or an empty X, gives NaN rather than a stray read
+    if ((x_pitch & 3) != 0 || x_pitch < dx || dx < 1 || n < 1) {
 #pragma unroll
       for (int k = 0; k < K; ++k) g[k] = T(NAN);
       lpu = (double)NAN;
       return 0.0;
     }
-    const int xp = x_pitch, tp = xt_pitch;
+    const int xp = x_pitch;
     T* beta = lds;                 // [K*64]
-    T* rbuf = lds + K * 64;        // [64V] residuals of a chunk
+    T* rbuf = lds + K * 64;        // [64V] residuals of a chunk (narrow)
     T* part = rbuf + 64 * V;       // [64 lanes][NB][V] gradient partials
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -579,16 +705,22 @@ struct Target {
     const int nbl = (dx + CPB - 1) / CPB;  // live blocks
     double lpl = 0.0;
     T ssq = T(0);
-    if (nbl <= 1 || NB == 1) {
-      glm_rows<1, LINEAR>(beta, rbuf, dx, xp, tp, inv_s2, cst, acc, lpl, ssq);
+    // narrow designs with X^T: fewer instructions per row in two passes
+    const bool narrow = nbl <= 1 && xt != nullptr && (xt_pitch & 3) == 0 &&
+                        xt_pitch >= n;
+    if (narrow) {
+      glm_rows<1, LINEAR>(beta, rbuf, dx, xp, xt_pitch, inv_s2, cst, acc, lpl,
+                          ssq);
+    } else if (nbl <= 1 || NB == 1) {
+      glm_pass<1, LINEAR>(beta, dx, xp, inv_s2, cst, acc, lpl, ssq);
     } else if (nbl == 2 || NB == 2) {
-      glm_rows<(NB >= 2 ? 2 : 1), LINEAR>(beta, rbuf, dx, xp, tp, inv_s2, cst,
-                                          acc, lpl, ssq);
+      glm_pass<(NB >= 2 ? 2 : 1), LINEAR>(beta, dx, xp, inv_s2, cst, acc, lpl,
+                                          ssq);
     } else if (nbl == 3 || NB == 3) {
-      glm_rows<(NB >= 3 ? 3 : 1), LINEAR>(beta, rbuf, dx, xp, tp, inv_s2, cst,
-                                          acc, lpl, ssq);
+      glm_pass<(NB >= 3 ? 3 : 1), LINEAR>(beta, dx, xp, inv_s2, cst, acc, lpl,
+                                          ssq);
     } else {
-      glm_rows<NB, LINEAR>(beta, rbuf, dx, xp, tp, inv_s2, cst, acc, lpl, ssq);
+      glm_pass<NB, LINEAR>(beta, dx, xp, inv_s2, cst, acc, lpl, ssq);
     }
     // sum the four row phases of each coefficient in a fixed order
 #pragma unroll

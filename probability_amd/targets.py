@@ -226,18 +226,34 @@ def _padded_rows(a, pitch_multiple=4):
   return buf, buf[:, :c]
 
 
+# Largest X (bytes) that also keeps a transposed copy: small designs stay
+# in L2, where a chain's leaf is bound by latency and instruction count
+# rather than by bytes.
+_XT_MAX_BYTES = 256 * 1024
+
+
 class _GlmData:
-  """X stored row-major and transposed, each with padded 16-byte rows."""
+  """X stored row-major with padded 16-byte rows.  Small designs of one
+  coefficient block (D <= 64 in float32, D <= 32 in float64; X at most
+  256 KiB) also keep X^T: the kernels then take the logits from X^T and the
+  gradient from X, fewer instructions per row.  Every other design gets
+  both from one pass over X, half the bytes."""
 
   def _set_design(self, x):
     if x.dim() != 2 or x.shape[0] < 1 or x.shape[1] < 1:
       raise ValueError('x must be a non-empty [N, D] matrix')
     self._xbuf, self.x = _padded_rows(x)
-    self._xtbuf, self.xt = _padded_rows(x.t())
+    self._xtbuf = self.xt = None
+    if (x.shape[1] <= 16 * (16 // x.element_size()) and
+        x.numel() * x.element_size() <= _XT_MAX_BYTES):
+      self._xtbuf, self.xt = _padded_rows(x.t())
 
   def _fill_design(self, p):
-    p.x, p.xt = self._xbuf.data_ptr(), self._xtbuf.data_ptr()
-    p.x_pitch, p.xt_pitch = self._xbuf.shape[1], self._xtbuf.shape[1]
+    p.x, p.x_pitch = self._xbuf.data_ptr(), self._xbuf.shape[1]
+    if self._xtbuf is None:
+      p.xt, p.xt_pitch = None, 0
+    else:
+      p.xt, p.xt_pitch = self._xtbuf.data_ptr(), self._xtbuf.shape[1]
 
 
 class LogisticRegression(_GlmData, TargetSpec):

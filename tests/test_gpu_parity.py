@@ -57,6 +57,34 @@ def _case(kind, dtype_np, seed=0, c=64):
     mk = lambda dt: ptg.LogisticRegression(x, y, 1.0, dtype=dt)
     q = cast(r.normal(size=(c, 100)) * 0.03)
     return o, mk, q, None
+  if kind == 'logistic_tall':
+    # 9000 x 8: one coefficient block, X too large to keep X^T (one pass)
+    x = cast(r.normal(size=(9000, 8)) * 0.1)
+    y = (r.uniform(size=9000) < 0.5).astype(np.float64)
+    o = otg.LogisticRegression(x, y, 1.0)
+    mk = lambda dt: ptg.LogisticRegression(x, y, 1.0, dtype=dt)
+    q = cast(r.normal(size=(c, 8)) * 0.05)
+    return o, mk, q, None
+  if kind == 'logistic_d150':
+    # 150 coefficients: three blocks per row in float32 (batches of 2 rows
+    # per phase), five in float64
+    x = cast(r.normal(size=(400, 150)) * 0.1)
+    y = (r.uniform(size=400) < 0.5).astype(np.float64)
+    o = otg.LogisticRegression(x, y, 1.0)
+    mk = lambda dt: ptg.LogisticRegression(x, y, 1.0, dtype=dt)
+    q = cast(r.normal(size=(c, 150)) * 0.05)
+    return o, mk, q, None
+  if kind == 'linear_wide':
+    # 70 coefficients: one pass over X (no X^T)
+    x = cast(r.normal(size=(900, 70)) * 0.3)
+    y = cast(x @ r.normal(size=70) * 0.2 + 0.5 * r.normal(size=900))
+    o = otg.LinearRegression(x, y, 1.0, 1.0)
+    mk = lambda dt: ptg.LinearRegression(x, y, 1.0, 1.0, dtype=dt)
+    q = cast(np.concatenate([r.normal(size=(c, 70)) * 0.05,
+                             np.full((c, 1), -0.5)], 1))
+    codes = np.zeros(71, np.int32)
+    codes[70] = 2  # Exp on sigma
+    return o, mk, q, codes
   if kind == 'linear_split':
     x = cast(r.normal(size=(1500, 11)))
     y = cast(x @ r.normal(size=11) * 0.3 + 0.5 * r.normal(size=1500))
@@ -159,7 +187,8 @@ def test_draws_match_oracle(tdt, ndt, tol):
 @pytest.mark.parametrize('tdt,ndt,tol', DTYPES)
 @pytest.mark.parametrize('kind', ['logistic', 'linear', 'hier', 'diag',
                                   'funnel', 'logistic_split', 'linear_split',
-                                  'logistic_wide'])
+                                  'logistic_wide', 'logistic_d150',
+                                  'linear_wide', 'logistic_tall'])
 def test_target_eval_matches_oracle(kind, tdt, ndt, tol):
   o, mk, q, codes = _case(kind, ndt)
   spec = mk(tdt)
@@ -247,7 +276,9 @@ def _nuts_one(kind, tdt, ndt, seed=0, c=64, eps=None, max_depth=10):
   ot = _otarget(o, codes)
   eps = eps or {'logistic': 0.08, 'linear': 0.05, 'hier': 0.3, 'diag': 0.5,
                 'funnel': 0.9, 'logistic_split': 0.04,
-                'linear_split': 0.02, 'logistic_wide': 0.02}[kind]
+                'linear_split': 0.02, 'logistic_wide': 0.02,
+                'logistic_d150': 0.05, 'linear_wide': 0.01,
+                'logistic_tall': 0.03}[kind]
   tspec = spec if codes is None else ptg.TransformedTarget(spec,
                                                            _bijs(spec, codes))
   kern = pmcmc.NoUTurnSampler(tspec, step_size=np.asarray(eps, ndt),
@@ -267,7 +298,8 @@ def _nuts_one(kind, tdt, ndt, seed=0, c=64, eps=None, max_depth=10):
 @pytest.mark.parametrize('tdt,ndt,tol', DTYPES)
 @pytest.mark.parametrize('kind', ['hier', 'logistic', 'linear', 'diag',
                                   'funnel', 'logistic_split', 'linear_split',
-                                  'logistic_wide'])
+                                  'logistic_wide', 'logistic_d150',
+                                  'linear_wide', 'logistic_tall'])
 def test_nuts_one_step_matches_oracle(kind, tdt, ndt, tol):
   for seed in range(3):
     spec, nxt, res, oq, ores = _nuts_one(kind, tdt, ndt, seed=seed)
