@@ -64,6 +64,7 @@ class Request:
     self.rows = None
     self.da_slot = None
     self.in_flight = False
+    self.waves = None
 
   @property
   def total_steps(self):
@@ -139,11 +140,13 @@ class _Pool:
 
 
 class _Lane:
-  """The launches of one (family, phase): a stream and the launch in flight
-  (completion event, its requests, buffers kept alive until it ends)."""
+  """The launches of one (family, waves per chain, phase): a stream and the
+  launch in flight (completion event, its requests, buffers kept alive until
+  it ends)."""
 
-  def __init__(self, family, adapting, device):
+  def __init__(self, family, waves, adapting, device):
     self.family = family
+    self.waves = waves
     self.adapting = adapting
     self.stream = torch.cuda.Stream(device=device)
     self.inflight = None
@@ -158,29 +161,45 @@ class SamplerServer:
     iters_per_tick: transitions a sampling-phase request makes per launch.
     slot_dims: register-slot class per family ({'glm': 192, 'hier': 512});
       every request of a family runs the same kernel code.
-    glm_waves_per_chain: waves sharing each GLM chain's gradient (fixed per
-      server, so a request's draws do not depend on the mix).
+    glm_waves_per_chain: waves sharing each GLM chain's gradient; None (the
+      default) takes each request's own from its N x D, a number fixes it
+      for every GLM request.  Either way a request's draws do not depend on
+      the mix: requests of one wave count share lanes (launches) of their
+      own, which run concurrently with the others.
   """
 
   def __init__(self, dtype=torch.float32, device='cuda',
                hbm_budget_bytes=32 << 30, max_chains=32768, max_requests=2048,
-               iters_per_tick=8, slot_dims=None, glm_waves_per_chain=4):
+               iters_per_tick=8, slot_dims=None, glm_waves_per_chain=None):
     _lib.require_cuda()
     self.dtype = dtype
     self.device = torch.device(device)
     self.budget = int(hbm_budget_bytes)
     self.used = 0
     self.iters_per_tick = int(iters_per_tick)
-    self.glm_waves_per_chain = int(glm_waves_per_chain)
+    self.glm_waves_per_chain = (None if glm_waves_per_chain is None
+                                else int(glm_waves_per_chain))
     slot_dims = slot_dims or {'glm': 192, 'hier': 512}
     self.pools = {f: _Pool(f, slot_dims[f], max_chains, max_requests, dtype,
                            self.device) for f in slot_dims}
     self.queue = collections.deque()
     self.active = []
     self.finished = []
-    self.lanes = [_Lane(f, ad, self.device) for f in slot_dims
-                  for ad in (True, False)]
+    self.lanes = {}
     self.ticks = 0
+
+  def _waves(self, req):
+    if _family(req.spec) != 'glm':
+      return 1
+    if self.glm_waves_per_chain is not None:
+      return self.glm_waves_per_chain
+    return _lib.glm_waves_per_chain(req.spec.n_obs * req.spec.dim)
+
+  def _lanes_for(self, req):
+    fam, w = _family(req.spec), req.waves
+    for ad in (True, False):
+      if (fam, w, ad) not in self.lanes:
+        self.lanes[(fam, w, ad)] = _Lane(fam, w, ad, self.device)
 
   # ------------------------------------------------------------ admission
   def submit(self, req):
@@ -199,6 +218,8 @@ class SamplerServer:
       self.queue.popleft()
       self.used += need
       pool.admit(req)
+      req.waves = self._waves(req)
+      self._lanes_for(req)
       req.admitted = time.perf_counter()
       req.draws = torch.empty((req.num_results, req.num_chains, req.spec.dim),
                               dtype=self.dtype, device=self.device)
@@ -270,8 +291,7 @@ class SamplerServer:
       a.unrolled_leapfrog_steps = 1
       a.target_kind = (_lib.TARGET_HIER_NORMAL if pool.family == 'hier'
                        else _lib.TARGET_LOGISTIC)
-      if pool.family == 'glm':
-        a.waves_per_chain = self.glm_waves_per_chain
+      a.waves_per_chain = reqs[0].waves
       a.max_energy_diff = 1000.0
       a.problems = table.data_ptr()
       a.n_problems = p
@@ -330,26 +350,27 @@ class SamplerServer:
       # the admitted rows were written on the current stream
       ev = torch.cuda.Event()
       ev.record()
-      for lane in self.lanes:
+      for lane in self.lanes.values():
         lane.stream.wait_event(ev)
     done = []
     while True:
       progressed = False
-      for lane in self.lanes:
+      for lane in self.lanes.values():
         if lane.inflight is not None:
           if not lane.inflight[0].query():
             continue
           done += self._complete(lane)
           progressed = True
         progressed |= self._start(lane)
-      busy = [l for l in self.lanes if l.inflight is not None]
+      busy = [l for l in self.lanes.values() if l.inflight is not None]
       if progressed or not busy:
         return done
       busy[0].inflight[0].synchronize()
 
   def _start(self, lane):
     reqs = [r for r in self.active if _family(r.spec) == lane.family and
-            r.adapting == lane.adapting and not r.in_flight]
+            r.waves == lane.waves and r.adapting == lane.adapting and
+            not r.in_flight]
     if not reqs:
       return False
     pool = self.pools[lane.family]
