@@ -208,6 +208,41 @@ __device__ inline void wave_sum2(A a, B b, A& ra, B& rb) {
   rb = readlane(b, 63);
 }
 
+// Four independent wave sums with their DPP stages interleaved (each sum
+// in the order of wave_sum).
+template <typename A>
+__device__ inline void wave_sum4(A a, A b, A c, A d, A& ra, A& rb, A& rc,
+                                 A& rd) {
+  a = dpp_add<kDppQuadXor1>(a);
+  b = dpp_add<kDppQuadXor1>(b);
+  c = dpp_add<kDppQuadXor1>(c);
+  d = dpp_add<kDppQuadXor1>(d);
+  a = dpp_add<kDppQuadXor2>(a);
+  b = dpp_add<kDppQuadXor2>(b);
+  c = dpp_add<kDppQuadXor2>(c);
+  d = dpp_add<kDppQuadXor2>(d);
+  a = dpp_add<kDppHalfMirror>(a);
+  b = dpp_add<kDppHalfMirror>(b);
+  c = dpp_add<kDppHalfMirror>(c);
+  d = dpp_add<kDppHalfMirror>(d);
+  a = dpp_add<kDppMirror>(a);
+  b = dpp_add<kDppMirror>(b);
+  c = dpp_add<kDppMirror>(c);
+  d = dpp_add<kDppMirror>(d);
+  a = dpp_add_rows<kDppRowBcast15, 0xA>(a);
+  b = dpp_add_rows<kDppRowBcast15, 0xA>(b);
+  c = dpp_add_rows<kDppRowBcast15, 0xA>(c);
+  d = dpp_add_rows<kDppRowBcast15, 0xA>(d);
+  a = dpp_add_rows<kDppRowBcast31, 0xC>(a);
+  b = dpp_add_rows<kDppRowBcast31, 0xC>(b);
+  c = dpp_add_rows<kDppRowBcast31, 0xC>(c);
+  d = dpp_add_rows<kDppRowBcast31, 0xC>(d);
+  ra = readlane(a, 63);
+  rb = readlane(b, 63);
+  rc = readlane(c, 63);
+  rd = readlane(d, 63);
+}
+
 // ---------------------------------------------------------------- scalars
 __device__ inline double neg_inf() { return -__builtin_inf(); }
 

@@ -229,40 +229,16 @@ __host__ __device__ constexpr size_t nuts_lds_elems(int max_depth) {
   return (size_t)2 * max_depth * K * 64 + target_scratch_elems<T, K, FAM>();
 }
 
-// GLM chains are bound by loads in flight, so their kernel is built for two
-// waves per SIMD (the register budget that allows) and blocks of up to
-// eight waves (one chain shared by a whole CU); the other families keep one
-// wave per SIMD and every register for the leaf's latency.
-template <typename T, int K, int FAM>
-__global__ void __launch_bounds__(FAM == kFamilyGlm ? 512 : 256)
-    __attribute__((amdgpu_waves_per_eu(FAM == kFamilyGlm ? 2 : 1)))
-    nuts_kernel(hipmc_run_args a) {
-  extern __shared__ __align__(16) unsigned char smem_raw[];
+// One chain's transitions (the body of nuts_kernel).  BM: the target's
+// bijector code path; U1: one leapfrog step per tree leaf
+// (unrolled_leapfrog_steps == 1).  Specialising both keeps the leaf free of
+// run-time branches and of the registers the other paths would hold.
+template <typename T, int K, int FAM, int BM, bool U1>
+__device__ __attribute__((always_inline)) inline void nuts_chain(
+    const hipmc_run_args& a, Target<T, K, FAM>& tg, const ChainId& id, int c,
+    int wic, T* memP, T* memC, const hipmc_problem& pb) {
   const int lane = lane_id();
-  const int wave = threadIdx.x >> 6;
-  const int waves = blockDim.x >> 6;
   const int max_depth = a.max_tree_depth;
-  // GLM chains may be shared by the W waves of a block (each evaluates a
-  // share of the rows, the partial sums meet in LDS, every wave runs the
-  // same control flow on the same values); otherwise a wave per chain.
-  const int W = waves_per_chain<FAM>(a);
-  const int c = W > 1 ? xcd_block(blockIdx.x, gridDim.x)
-                      : xcd_block(blockIdx.x, gridDim.x) * waves + wave;
-  const int wic = W > 1 ? wave : 0;
-  if (c >= a.n_chains) return;  // whole wave (block, if shared) leaves
-  const size_t umem = (size_t)2 * max_depth * K * 64;
-  const size_t scr = target_scratch_elems<T, K, FAM>();
-  T* wl = W > 1 ? (T*)smem_raw
-                : (T*)smem_raw + (size_t)wave * nuts_lds_elems<T, K, FAM>(max_depth);
-  T* memP = wl;  // one copy per chain; the waves of a chain write equal values
-  T* memC = wl + (size_t)max_depth * K * 64;
-  T* scratch = wl + umem + (size_t)wic * scr;
-
-  const ChainId id = chain_of(a, c);
-  const hipmc_problem& pb = a.problems[id.prob];
-  Target<T, K, FAM> tg;
-  tg.load(pb, scratch);
-  if (W > 1) tg.set_split(wic, W, (unsigned char*)(wl + umem + (size_t)W * scr));
   const int dim = pb.dim;
   const int crow = a.chain_row ? a.chain_row[c] : c;
   const size_t row = (size_t)crow * a.dim_stride;
@@ -340,7 +316,7 @@ __global__ void __launch_bounds__(FAM == kFamilyGlm ? 512 : 256)
           lu_buf = (double)log1m(uniform_from<T>(lw.x, lw.y));
         }
         double energy;
-        if (nunroll == 1) {
+        if (U1 || nunroll == 1) {
           const T half = T(0.5) * e;
 #pragma unroll
           for (int k = 0; k < K; ++k) {
@@ -348,7 +324,7 @@ __global__ void __launch_bounds__(FAM == kFamilyGlm ? 512 : 256)
             q[k] = q[k] + e * p[k];
           }
           double lpu;
-          const double lpl = tg.eval_partial(q, g, lpu);
+          const double lpl = tg.template eval_partial<BM>(q, g, lpu);
           double kin = 0.0;
 #pragma unroll
           for (int k = 0; k < K; ++k) {
@@ -385,21 +361,31 @@ __global__ void __launch_bounds__(FAM == kFamilyGlm ? 512 : 256)
           __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
           __builtin_amdgcn_wave_barrier();
           __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+          // Two checks per round, their reductions interleaved; the result
+          // is the AND of the same per-slot tests (a lone last check is
+          // evaluated twice).
           const int top = __popc(i);
           const int lo = top - __builtin_ctz((unsigned)(i + 1));
-          for (int j = lo; j < top && no_u; ++j) {
-            const T* mp = memP + (size_t)j * K * 64;
-            const T* mc = memC + (size_t)j * K * 64;
-            T sl = T(0), sr = T(0);
+          for (int j = lo; j < top && no_u; j += 2) {
+            const int j1 = j + 1 < top ? j + 1 : j;
+            const T* mp0 = memP + (size_t)j * K * 64;
+            const T* mc0 = memC + (size_t)j * K * 64;
+            const T* mp1 = memP + (size_t)j1 * K * 64;
+            const T* mc1 = memC + (size_t)j1 * K * 64;
+            T sl0 = T(0), sr0 = T(0), sl1 = T(0), sr1 = T(0);
 #pragma unroll
             for (int k = 0; k < K; ++k) {
-              const T d = cs[k] - mc[k * 64 + lane];
-              sl += d * mp[k * 64 + lane];
-              sr += d * p[k];
+              const T d0 = cs[k] - mc0[k * 64 + lane];
+              const T d1 = cs[k] - mc1[k * 64 + lane];
+              sl0 += d0 * mp0[k * 64 + lane];
+              sr0 += d0 * p[k];
+              sl1 += d1 * mp1[k * 64 + lane];
+              sr1 += d1 * p[k];
             }
-            T dl, dr;
-            wave_sum2(sl, sr, dl, dr);
-            no_u = (dl >= T(0)) && (dr >= T(0));
+            T dl0, dr0, dl1, dr1;
+            wave_sum4(sl0, sr0, sl1, sr1, dl0, dr0, dl1, dr1);
+            no_u = (dl0 >= T(0)) && (dr0 >= T(0)) && (dl1 >= T(0)) &&
+                   (dr1 >= T(0));
           }
         }
         if (__builtin_isnan(energy)) energy = neg_inf();
@@ -473,6 +459,50 @@ __global__ void __launch_bounds__(FAM == kFamilyGlm ? 512 : 256)
   store_row((T*)a.state + row, q0, dim);
   store_row((T*)a.grad + row, g0, dim);
   if (lane == 0) ((T*)a.tlp)[crow] = (T)t0;
+}
+
+// GLM chains are bound by loads in flight, so their kernel is built for two
+// waves per SIMD (the register budget that allows) and blocks of up to
+// eight waves (one chain shared by a whole CU); the other families keep one
+// wave per SIMD and every register for the leaf's latency.
+template <typename T, int K, int FAM>
+__global__ void __launch_bounds__(FAM == kFamilyGlm ? 512 : 256)
+    __attribute__((amdgpu_waves_per_eu(FAM == kFamilyGlm ? 2 : 1)))
+    nuts_kernel(hipmc_run_args a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int wave = threadIdx.x >> 6;
+  const int waves = blockDim.x >> 6;
+  const int max_depth = a.max_tree_depth;
+  // GLM chains may be shared by the W waves of a block (each evaluates a
+  // share of the rows, the partial sums meet in LDS, every wave runs the
+  // same control flow on the same values); otherwise a wave per chain.
+  const int W = waves_per_chain<FAM>(a);
+  const int c = W > 1 ? xcd_block(blockIdx.x, gridDim.x)
+                      : xcd_block(blockIdx.x, gridDim.x) * waves + wave;
+  const int wic = W > 1 ? wave : 0;
+  if (c >= a.n_chains) return;  // whole wave (block, if shared) leaves
+  const size_t umem = (size_t)2 * max_depth * K * 64;
+  const size_t scr = target_scratch_elems<T, K, FAM>();
+  T* wl = W > 1 ? (T*)smem_raw
+                : (T*)smem_raw + (size_t)wave * nuts_lds_elems<T, K, FAM>(max_depth);
+  T* memP = wl;  // one copy per chain; the waves of a chain write equal values
+  T* memC = wl + (size_t)max_depth * K * 64;
+  T* scratch = wl + umem + (size_t)wic * scr;
+
+  const ChainId id = chain_of(a, c);
+  const hipmc_problem& pb = a.problems[id.prob];
+  Target<T, K, FAM> tg;
+  tg.load(pb, scratch);
+  if (W > 1) tg.set_split(wic, W, (unsigned char*)(wl + umem + (size_t)W * scr));
+  const bool u1 = a.unrolled_leapfrog_steps == 1;
+  if (FAM == kFamilyHier && u1 && tg.sp0) {
+    nuts_chain<T, K, FAM, kBijSlot0Softplus, true>(a, tg, id, c, wic, memP,
+                                                    memC, pb);
+  } else if (u1) {
+    nuts_chain<T, K, FAM, kBijAny, true>(a, tg, id, c, wic, memP, memC, pb);
+  } else {
+    nuts_chain<T, K, FAM, kBijAny, false>(a, tg, id, c, wic, memP, memC, pb);
+  }
 }
 
 // ------------------------------------------------------------------ HMC

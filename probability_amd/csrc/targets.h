@@ -106,6 +106,10 @@ __device__ inline void put_slot(T (&v)[K], int s, T f) {
   }
 }
 
+// Bijector code paths of Target::eval_partial.
+constexpr int kBijAny = 0;            // chosen at run time from the layout
+constexpr int kBijSlot0Softplus = 1;  // every bijector lane: Softplus, slot 0
+
 template <typename T, int K, int FAM>
 struct Target {
   int kind, dim, n_obs;
@@ -203,6 +207,10 @@ struct Target {
   // Per-lane partial log-prob (double) and wave-uniform part `lpu`; writes
   // the gradient.  The caller reduces the partial (possibly together with
   // other sums).  Bijector log-det-Jacobians are folded into the partial.
+  // BM = kBijSlot0Softplus compiles only the path for bijectors that are all
+  // Softplus in register slot 0 (the caller has checked sp0); kBijAny
+  // chooses the path at run time.
+  template <int BM = kBijAny>
   __device__ double eval_partial(const T (&xs)[K], T (&g)[K], double& lpu) {
     T th[K];
     SoftplusParts<T> sp[K];
@@ -213,10 +221,11 @@ struct Target {
     // branches); otherwise the per-slot loop
     SoftplusParts<T> spb{};
     T exb = T(0);
-    if (sp0) {
+    if (BM == kBijSlot0Softplus || sp0) {
       spb = softplus_parts(xs[0]);
       th[0] = code[0] == HIPMC_BIJ_SOFTPLUS ? fmax(xs[0], T(0)) + spb.l
                                             : xs[0];
+    } else if (BM == kBijSlot0Softplus) {
     } else if (one_slot) {
       const T xb = pick(xs, bslot);
       if (sp_slots) spb = softplus_parts(xb);
@@ -259,10 +268,11 @@ struct Target {
     }
     // fldj: Softplus -softplus(-x) with d/dx f = sigmoid(x),
     // d/dx fldj = sigmoid(-x); Exp x with d/dx f = exp(x), d/dx fldj = 1
-    if (sp0) {
+    if (BM == kBijSlot0Softplus || sp0) {
       const bool on = code[0] == HIPMC_BIJ_SOFTPLUS;
       lpl += on ? (double)(-(fmax(-xs[0], T(0)) + spb.l)) : 0.0;
       g[0] = on ? g[0] * spb.sig + spb.sig_neg : g[0];
+    } else if (BM == kBijSlot0Softplus) {
     } else if (one_slot) {
       const T xb = pick(xs, bslot);
       const T gb = pick(g, bslot);
@@ -294,9 +304,10 @@ struct Target {
   }
 
   // Wave-uniform target log-prob (double); writes the gradient.
+  template <int BM = kBijAny>
   __device__ double eval(const T (&xs)[K], T (&g)[K]) {
     double lpu;
-    const double lpl = eval_partial(xs, g, lpu);
+    const double lpl = eval_partial<BM>(xs, g, lpu);
     return wave_sum(lpl) + lpu;
   }
 
