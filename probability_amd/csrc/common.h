@@ -100,6 +100,18 @@ __device__ inline double readlane_d(double v, int l) {
 }
 __device__ inline float readlane(float v, int l) { return readlane_f(v, l); }
 __device__ inline double readlane(double v, int l) { return readlane_d(v, l); }
+// A value every lane already holds, marked wave-uniform (a scalar register):
+// branches on it become scalar branches and selects on it plain v_cndmask,
+// instead of exec-mask regions.
+__device__ inline float uniform(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ inline double uniform(double v) {
+  const int64_t b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readfirstlane((int)(uint32_t)b);
+  const int hi = __builtin_amdgcn_readfirstlane((int)(uint32_t)(b >> 32));
+  return __longlong_as_double(((int64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
 
 // v + v[perm(lane)] for a DPP permutation inside each row of 16 lanes.  The
 // float form compiles to one v_add_f32_dpp.

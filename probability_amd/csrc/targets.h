@@ -95,6 +95,16 @@ __device__ inline T pick(const T (&v)[K], int s) {
   for (int k = 0; k < K; ++k) r |= Bits<T>::of(v[k]) & (s == k ? ~U(0) : U(0));
   return Bits<T>::to(r);
 }
+// lane 0 ? a : (lane 1 ? b : v), blended with integer masks so that v, a
+// and b are all computed in every lane (no exec-mask region).
+template <typename T>
+__device__ inline T lanes01(T v, T a, T b, int lane) {
+  using U = typename Bits<T>::type;
+  const U m0 = lane == 0 ? ~U(0) : U(0);
+  const U m1 = lane == 1 ? ~U(0) : U(0);
+  return Bits<T>::to((Bits<T>::of(v) & ~(m0 | m1)) | (Bits<T>::of(a) & m0) |
+                     (Bits<T>::of(b) & m1));
+}
 template <typename T, int K>
 __device__ inline void put_slot(T (&v)[K], int s, T f) {
   using U = typename Bits<T>::type;
@@ -334,12 +344,14 @@ struct Target {
     }
     T S1, S2;
     wave_sum2(s1, s2, S1, S2);
-    const T gmu = -mu * inv_s0sq + S1;
-    const T gtau = T(-2) * tau * recip(s1sq + tau * tau) + S2;
-    g[0] = lane == 0 ? gmu : (lane == 1 ? gtau : g[0]);
+    // wave-uniform scalars: computed once, then blended into lanes 0 and 1
+    const T gmu = uniform(-mu * inv_s0sq + S1);
+    const T gtau = uniform(T(-2) * tau * recip(s1sq + tau * tau) + S2);
+    g[0] = lanes01(g[0], gmu, gtau, lane);
     const T zm = mu * inv_s0;
     const T zt = tau * inv_s1;
-    lpu = log_norm + (double)(T(-0.5) * zm * zm) - (double)log1p_pos(zt * zt);
+    lpu = log_norm + (double)(T(-0.5) * zm * zm) -
+          (double)log1p_pos(uniform(zt * zt));
     if (tau < T(0)) lpu = neg_inf();
     return lpl;
   }
