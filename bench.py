@@ -171,6 +171,12 @@ def main():
   import torch
   import probability_amd as pa
   from probability_amd import workloads
+  if args.calib_copy:
+    src = torch.ones(64 << 20, device='cuda')
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
   if args.workload != 'hier_normal_j256':
     import bench_workloads
     line = bench_workloads.run(args, torch, pa)
@@ -182,12 +188,6 @@ def main():
     wl['num_adaptation_steps'] = wl['num_burnin_steps'] = args.adapt
   if args.draws is not None:
     wl['num_results'] = args.draws
-  if args.calib_copy:
-    src = torch.ones(64 << 20, device='cuda')
-    dst = torch.empty_like(src)
-    dst.copy_(src)
-    torch.cuda.synchronize()
-    del src, dst
   j, c = len(wl['y']), wl['chains']
   spec, da, init = build_kernel(wl, torch, pa)
 

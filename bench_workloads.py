@@ -155,7 +155,8 @@ def _cpu_hmc(wl, min_ess, budget_s):
 def logistic_65536x256(args, torch, pa):
   from probability_amd import workloads
   n, d, c, L = 65536, 256, 4096, 20
-  nb, nr = 500, 1000
+  nb = 500 if args.adapt is None else args.adapt
+  nr = 1000 if args.draws is None else args.draws
   x, y, _ = workloads.logistic_data(n, d, seed=5, x_scale=1.0 / 16.0)
   spec = pa.targets.LogisticRegressionBF16(x, y, device='cuda')
   hmc = pa.mcmc.HamiltonianMonteCarlo(spec, step_size=np.float32(0.02),
