@@ -27,8 +27,6 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 constexpr int kMD = 256;       // state dimension of this path
 constexpr int kMRows = 32;     // rows per LDS tile
 constexpr int kMChains = 64;   // chains per workgroup (16 per wave)
-constexpr int kXsPitch = kMD + 8;      // bf16 per X-tile row in LDS
-constexpr int kXtPitch = kMRows + 8;   // bf16 per X^T-tile row in LDS
 constexpr int kRPitch = kMRows + 8;    // bf16 per R row (one chain) in LDS
 
 __device__ inline f32x4 mfma16(const bf16x8& a, const bf16x8& b,
@@ -36,8 +34,57 @@ __device__ inline f32x4 mfma16(const bf16x8& a, const bf16x8& b,
   return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
 }
 
+// Direct global -> LDS loads (global_load_lds): the wave's 64 lanes each
+// read 16 (or 4) bytes from their own address and the bytes land at the
+// wave-uniform LDS base + 16 (4) * lane.  Issued from inline asm so that the
+// compiler's wait-count pass does not make every later LDS read wait for
+// them; the kernel waits for its own tile loads (vmcnt) before the barrier
+// that publishes a tile.
+typedef __attribute__((address_space(3))) char lds_byte;
+__device__ inline uint32_t lds_addr(void* p) {
+  return (uint32_t)(uintptr_t)(lds_byte*)p;
+}
+__device__ inline void lds_dma16(const void* g, void* lds_base) {
+  asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dwordx4 %0, off"
+               :: "v"(g), "s"(lds_addr(lds_base)) : "memory", "m0");
+}
+__device__ inline void lds_dma4(const void* g, void* lds_base) {
+  asm volatile("s_mov_b32 m0, %1\n\tglobal_load_lds_dword %0, off"
+               :: "v"(g), "s"(lds_addr(lds_base)) : "memory", "m0");
+}
+
+// X tile: 16-byte chunk (row, c) at chunk row * 32 + (c ^ (row & 7));
+// X^T tile: chunk (d, c) at d * 4 + (c ^ ((d >> 1) & 3)).
+__device__ inline void stage_tile(const __bf16* X, const __bf16* Xt,
+                                  const float* y, int n_obs, int n0,
+                                  __bf16* xs, __bf16* xts, float* ys, int w,
+                                  int l) {
+  w = __builtin_amdgcn_readfirstlane(w);
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int q = (w * 4 + u) * 64 + l;
+    const int row = q >> 5, c = (q & 31) ^ (row & 7);
+    lds_dma16(X + (size_t)(n0 + row) * kMD + c * 8,
+              (char*)xs + (size_t)(w * 4 + u) * 1024);
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int q = (w * 4 + u) * 64 + l;
+    const int d = q >> 2, c = (q & 3) ^ ((d >> 1) & 3);
+    lds_dma16(Xt + (size_t)d * n_obs + n0 + c * 8,
+              (char*)xts + (size_t)(w * 4 + u) * 1024);
+  }
+  if (l < 8) lds_dma4(y + n0 + w * 8 + l, ys + w * 8);
+}
+
 // Partial gradient (and optionally partial Bernoulli log-likelihood) of the
 // chains [64*blockIdx.x, +64) over rows [blockIdx.y*rows_per_split, +rows).
+// Tiles of 32 rows of X and X^T go straight from memory to LDS (no
+// registers) into one of two buffers while the previous tile is used, so a
+// tile's load latency hides behind the previous tile's MFMAs.  LDS layouts
+// (16-byte chunks, XOR-swizzled so that the MFMA fragment reads of 16 rows
+// hit distinct banks): X tile chunk (row, c) at row * 32 + (c ^ (row & 7)),
+// X^T tile chunk (d, c) at d * 4 + (c ^ ((d >> 1) & 3)).
 __global__ void __launch_bounds__(256, 2)
     glm_grad_mfma(const __bf16* __restrict__ X, const __bf16* __restrict__ Xt,
                   const float* __restrict__ y, int n_obs,
@@ -45,15 +92,16 @@ __global__ void __launch_bounds__(256, 2)
                   const __bf16* __restrict__ Blo, int n_chains,
                   int rows_per_split, float* __restrict__ out_g,
                   float* __restrict__ out_ll) {
-  __shared__ __align__(16) __bf16 xs[kMRows * kXsPitch];
-  __shared__ __align__(16) __bf16 xts[kMD * kXtPitch];
+  __shared__ __align__(1024) __bf16 xs0[kMRows * kMD];
+  __shared__ __align__(1024) __bf16 xs1[kMRows * kMD];
+  __shared__ __align__(1024) __bf16 xts0[kMD * kMRows];
+  __shared__ __align__(1024) __bf16 xts1[kMD * kMRows];
   __shared__ __align__(16) __bf16 rs[2 * kMChains * kRPitch];
-  __shared__ float ys[kMRows];
+  __shared__ __align__(16) float ys0[kMRows];
+  __shared__ __align__(16) float ys1[kMRows];
   const int tid = threadIdx.x;
   const int w = tid >> 6, l = tid & 63, m = l & 15, g = l >> 4;
   const int c0 = blockIdx.x * kMChains + 16 * w;
-
-  // B operand of S = X B^T: lane holds chain c0+m, d = 32*ks + 8*g + [0, 8).
   bf16x8 bh[kMD / 32], bl[kMD / 32];
 #pragma unroll
   for (int ks = 0; ks < kMD / 32; ++ks) {
@@ -61,65 +109,32 @@ __global__ void __launch_bounds__(256, 2)
     bh[ks] = *(const bf16x8*)(Bhi + off);
     bl[ks] = *(const bf16x8*)(Blo + off);
   }
+  // keep the fragments in registers (not re-read from memory per tile,
+  // whose waits would also wait for the tile loads in flight)
+#pragma unroll
+  for (int ks = 0; ks < kMD / 32; ++ks) asm volatile("" : "+v"(bh[ks]), "+v"(bl[ks]));
   f32x4 acc[kMD / 16];
 #pragma unroll
   for (int nt = 0; nt < kMD / 16; ++nt) acc[nt] = f32x4{0.f, 0.f, 0.f, 0.f};
   float lls = 0.f;
   const bool want_ll = out_ll != nullptr;
-
   const int r0 = blockIdx.y * rows_per_split;
   const int r1 = min(n_obs, r0 + rows_per_split);
   __bf16* rh_w = rs + (16 * w) * kRPitch;
   __bf16* rl_w = rs + kMChains * kRPitch + (16 * w) * kRPitch;
-  for (int n0 = r0; n0 < r1; n0 += kMRows) {
-    __syncthreads();
-    // stage X[n0:n0+32, :] and X^T[:, n0:n0+32] (16-byte chunks): all eight
-    // loads of a thread are issued before the first LDS store, so a tile
-    // costs one global round trip
-    constexpr int kChunksX = kMRows * (kMD / 8) / 256;
-    constexpr int kChunksT = kMD * (kMRows / 8) / 256;
-    uint4 px[kChunksX], pt[kChunksT];
-#pragma unroll
-    for (int u = 0; u < kChunksX; ++u) {
-      const int i = tid + 256 * u;
-      const int row = i / (kMD / 8), ch = i % (kMD / 8);
-      px[u] = *(const uint4*)(X + (size_t)(n0 + row) * kMD + ch * 8);
-    }
-#pragma unroll
-    for (int u = 0; u < kChunksT; ++u) {
-      const int i = tid + 256 * u;
-      const int d = i / (kMRows / 8), ch = i % (kMRows / 8);
-      pt[u] = *(const uint4*)(Xt + (size_t)d * n_obs + n0 + ch * 8);
-    }
-    const float py = tid < kMRows ? y[n0 + tid] : 0.f;
-#pragma unroll
-    for (int u = 0; u < kChunksX; ++u) {
-      const int i = tid + 256 * u;
-      const int row = i / (kMD / 8), ch = i % (kMD / 8);
-      *(uint4*)&xs[row * kXsPitch + ch * 8] = px[u];
-    }
-#pragma unroll
-    for (int u = 0; u < kChunksT; ++u) {
-      const int i = tid + 256 * u;
-      const int d = i / (kMRows / 8), ch = i % (kMRows / 8);
-      *(uint4*)&xts[d * kXtPitch + ch * 8] = pt[u];
-    }
-    if (tid < kMRows) ys[tid] = py;
-    __syncthreads();
 
-    // S tile [32 rows x 16 chains] of this wave: two 16x16 blocks.
+  auto compute = [&](const __bf16* xs, const __bf16* xts, const float* ys) {
     f32x4 s0 = f32x4{0.f, 0.f, 0.f, 0.f}, s1 = s0;
 #pragma unroll
     for (int ks = 0; ks < kMD / 32; ++ks) {
-      const bf16x8 a0 = *(const bf16x8*)&xs[m * kXsPitch + 32 * ks + 8 * g];
-      const bf16x8 a1 =
-          *(const bf16x8*)&xs[(16 + m) * kXsPitch + 32 * ks + 8 * g];
+      const int c = 4 * ks + g;
+      const bf16x8 a0 = *(const bf16x8*)&xs[(m * 32 + (c ^ (m & 7))) * 8];
+      const bf16x8 a1 = *(const bf16x8*)&xs[((16 + m) * 32 + (c ^ (m & 7))) * 8];
       s0 = mfma16(a0, bh[ks], s0);
       s1 = mfma16(a1, bh[ks], s1);
       s0 = mfma16(a0, bl[ks], s0);
       s1 = mfma16(a1, bl[ks], s1);
     }
-    // R = y - sigmoid(S); lane holds rows 16*mt + 4*g + i of chain c0+m.
 #pragma unroll
     for (int mt = 0; mt < 2; ++mt) {
 #pragma unroll
@@ -132,10 +147,9 @@ __global__ void __launch_bounds__(256, 2)
         const float sig = sv >= 0.f ? inv : e * inv;
         const float r = yv - sig;
         if (want_ll) {
-          // multiply_no_nan(-softplus(s), 1-y) + multiply_no_nan(-softplus(-s), y)
           const float sp_abs = log1p_unit(e);
-          const float sp_pos = sp_abs + fmaxf(sv, 0.f);   // softplus(s)
-          const float sp_neg = sp_abs + fmaxf(-sv, 0.f);  // softplus(-s)
+          const float sp_pos = sp_abs + fmaxf(sv, 0.f);
+          const float sp_neg = sp_abs + fmaxf(-sv, 0.f);
           const float w0 = 1.f - yv;
           lls += (w0 == 0.f ? 0.f : -sp_pos * w0) + (yv == 0.f ? 0.f : -sp_neg * yv);
         }
@@ -148,24 +162,51 @@ __global__ void __launch_bounds__(256, 2)
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    // G[16 chains x 256] += R^T[16 x 32 rows] X[32 rows x 256]
     const bf16x8 ah = *(const bf16x8*)&rh_w[m * kRPitch + 8 * g];
     const bf16x8 al = *(const bf16x8*)&rl_w[m * kRPitch + 8 * g];
-    // four B fragments are read from LDS ahead of their eight MFMAs
 #pragma unroll
     for (int nt0 = 0; nt0 < kMD / 16; nt0 += 4) {
       bf16x8 b[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u)
-        b[u] = *(const bf16x8*)&xts[(16 * (nt0 + u) + m) * kXtPitch + 8 * g];
+      for (int u = 0; u < 4; ++u) {
+        const int d = 16 * (nt0 + u) + m;
+        b[u] = *(const bf16x8*)&xts[(d * 4 + (g ^ ((d >> 1) & 3))) * 8];
+      }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         acc[nt0 + u] = mfma16(ah, b[u], acc[nt0 + u]);
         acc[nt0 + u] = mfma16(al, b[u], acc[nt0 + u]);
       }
     }
+  };
+  // tiles of this split, two per round (buffers 0 and 1): the next tile's
+  // nine loads per wave are issued, then the wave waits until only those
+  // are outstanding (vmcnt(9)), i.e. until its loads of the current tile
+  // have landed; the barrier makes every wave's part visible
+  if (r0 < r1) stage_tile(X, Xt, y, n_obs, r0, xs0, xts0, ys0, w, l);
+  for (int n0 = r0; n0 < r1; n0 += 2 * kMRows) {
+    const int n1 = n0 + kMRows;
+    if (n1 < r1) {
+      stage_tile(X, Xt, y, n_obs, n1, xs1, xts1, ys1, w, l);
+      __builtin_amdgcn_s_waitcnt(0xF79);  // vmcnt(9): tile n0 has landed
+    } else {
+      __builtin_amdgcn_s_waitcnt(0xF70);  // vmcnt(0)
+    }
+    __syncthreads();
+    compute(xs0, xts0, ys0);
+    __syncthreads();
+    if (n1 >= r1) break;
+    const int n2 = n1 + kMRows;
+    if (n2 < r1) {
+      stage_tile(X, Xt, y, n_obs, n2, xs0, xts0, ys0, w, l);
+      __builtin_amdgcn_s_waitcnt(0xF79);
+    } else {
+      __builtin_amdgcn_s_waitcnt(0xF70);
+    }
+    __syncthreads();
+    compute(xs1, xts1, ys1);
+    __syncthreads();
   }
-  // lane holds G[chain c0 + 4g + i][16*nt + m]
   float* og = out_g + ((size_t)blockIdx.y * n_chains + c0) * kMD;
 #pragma unroll
   for (int nt = 0; nt < kMD / 16; ++nt)
@@ -177,7 +218,6 @@ __global__ void __launch_bounds__(256, 2)
     if (g == 0) out_ll[(size_t)blockIdx.y * n_chains + c0 + m] = lls;
   }
 }
-
 // ---------------------------------------------------------------- per chain
 // One wave per chain, element e in lane e % 64, slot e / 64 (4 slots).
 __device__ inline void split_bf16(float v, __bf16& hi, __bf16& lo) {
