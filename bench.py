@@ -40,9 +40,9 @@ sys.path.insert(0, ROOT)
 
 FP32_PEAK_TFLOPS = 157.3
 # HBM bytes of one adaptation launch of nuts_kernel on this workload:
-# rocprofv3 FETCH_SIZE (1390 KiB) x 2.00 + WRITE_SIZE (4643 KiB) x 1.00, the
+# rocprofv3 FETCH_SIZE (1330 KiB) x 2.00 + WRITE_SIZE (2851 KiB) x 1.00, the
 # factors being what the two counters read for a 256 MiB device copy.
-NUTS_LAUNCH_TRAFFIC_BYTES = (1390 * 2 + 4643) * 1024
+NUTS_LAUNCH_TRAFFIC_BYTES = (1330 * 2 + 2851) * 1024
 # Algorithmic FP32 work of one chain-leapfrog on the hierarchical normal:
 # 13 FLOP per group for log-prob + gradient (theta, residual, two scalings,
 # the two quadratic terms, grad_z, the two reductions), 8 FLOP per state
