@@ -24,6 +24,9 @@
  *                        (mcmc/transformed_kernel.py:86-128).
  *   hipmc_da_update      DualAveragingStepSizeAdaptation._one_step_part
  *                        (mcmc/dual_averaging_step_size_adaptation.py:353-475).
+ *   hipmc_simple_step_update
+ *                        SimpleStepSizeAdaptation.one_step's step-size rule
+ *                        (mcmc/simple_step_size_adaptation.py:344-438).
  *   hipmc_mh_accept      MetropolisHastings accept/reject
  *                        (mcmc/metropolis_hastings.py:204-246).
  *   hipmc_leapfrog_run   UncalibratedHamiltonianMonteCarlo.one_step proposal
@@ -323,6 +326,27 @@ int hipmc_da_update(int32_t dtype, int32_t n_steps, const int32_t* step_group,
                     double step_count_smoothing, double decay_rate,
                     void* step_size, double* new_step_size_f64,
                     uint64_t stream);
+
+/*
+ * SimpleStepSizeAdaptation update of n_elems step-size elements over n_rows
+ * successive steps (row r reads log_accept_ratio + r*row_stride and is step
+ * `step + r`).  Element s reads the chains of group g = step_group[s] (NULL:
+ * g = s), entries [group_start[g], group_start[g+1]) (dtype, contiguous),
+ * whose log-mean-exp of min(0, lar) (non-finite lar as -inf) is taken in
+ * float64; while step + r < num_adaptation_steps the element is multiplied
+ * by (1 + adaptation_rate) if that exceeds log(target_accept_prob[s]) and
+ * divided by it otherwise, in the step-size dtype.  step_size: dtype
+ * [n_elems], read and written; target_accept_prob: float64 [n_elems].
+ */
+int hipmc_simple_step_update(int32_t dtype, int32_t n_elems,
+                             const int32_t* step_group,
+                             const int32_t* group_start,
+                             const void* log_accept_ratio, int32_t n_rows,
+                             int64_t row_stride,
+                             const double* target_accept_prob,
+                             double adaptation_rate, int64_t step,
+                             int64_t num_adaptation_steps, void* step_size,
+                             uint64_t stream);
 
 /*
  * Metropolis-Hastings accept for n_chains chains:

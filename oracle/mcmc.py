@@ -394,6 +394,35 @@ def da_update_grouped(state, log_accept_ratio, groups, n_groups,
 
 
 # --------------------------------------------------------------------------
+# Simple step-size adaptation  (`mcmc/simple_step_size_adaptation.py:344-438`)
+
+def simple_step_update(step_size, log_accept_ratio, state_rank, step,
+                       num_adaptation_steps, target_accept_prob=0.75,
+                       adaptation_rate=0.01, reduce_fn=reduce_logmeanexp):
+  """New value of one step-size part after the inner step number `step`.
+
+  The chains' log accept probabilities are reduced over the leading
+  min(rank(lap), state_rank - rank(step)) dimensions, then over the
+  dimensions where they and the step size differ (kept); the reduced value
+  aligns with the step size from the left (`mcmc_util.choose`).  Above
+  log(target) the step grows by (1 + rate), otherwise it shrinks by it; from
+  step num_adaptation_steps on it stays (`:410-436`).
+  """
+  lap = log_accept_prob(log_accept_ratio)
+  step_size = np.asarray(step_size, np.float64)
+  num_reduce = min(lap.ndim, state_rank - step_size.ndim)
+  red = reduce_fn(lap, axis=tuple(range(num_reduce)), keepdims=False)
+  idx = _differing_dims(np.shape(red), step_size.shape)
+  red = reduce_fn(red, axis=tuple(idx), keepdims=True) if idx else red
+  red = np.asarray(red)
+  red = np.reshape(red, red.shape + (1,) * max(step_size.ndim - red.ndim, 0))
+  up = red > np.log(np.asarray(target_accept_prob, np.float64))
+  opr = 1.0 + adaptation_rate
+  new = np.where(up, step_size * opr, step_size / opr)
+  return new if step < num_adaptation_steps else step_size
+
+
+# --------------------------------------------------------------------------
 # sample_chain drivers  (`mcmc/sample.py:311-383`)
 
 

@@ -27,7 +27,8 @@ EXPORTED_SYMBOLS = (
     'hipmc_sizeof_glm_hmc_args', 'hipmc_glm_hmc_run', 'hipmc_seed_split', 'hipmc_seed_fold_in',
     'hipmc_seed_salt', 'hipmc_philox_host', 'hipmc_hmc_run',
     'hipmc_leapfrog_run', 'hipmc_nuts_run', 'hipmc_target_eval',
-    'hipmc_da_update', 'hipmc_mh_accept', 'hipmc_bijector', 'hipmc_draws')
+    'hipmc_da_update', 'hipmc_simple_step_update', 'hipmc_mh_accept',
+    'hipmc_bijector', 'hipmc_draws')
 
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32
@@ -124,6 +125,9 @@ def lib():
       l.hipmc_da_update.argtypes = [_i32, _i32, _vp, _vp, _vp, _vp, _i32, _i64, _vp, _vp, _vp, _vp,
                                     _i64, _i64, _f64, _f64, _f64, _vp, _vp,
                                     _u64]
+      l.hipmc_simple_step_update.argtypes = [_i32, _i32, _vp, _vp, _vp, _i32,
+                                             _i64, _vp, _f64, _i64, _i64, _vp,
+                                             _u64]
       l.hipmc_mh_accept.argtypes = [_i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp,
                                     _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                     _u64]

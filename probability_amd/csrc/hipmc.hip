@@ -227,6 +227,42 @@ struct EvalL {
 };
 
 // ------------------------------------------------------------ DA kernel
+// reduce_logmeanexp of hmc_like_log_accept_prob_getter_fn(lar) over chains
+// [b, e) (`simple_step_size_adaptation.py:42-48`, `math/generic.py:221-273`),
+// float64; every thread of the block returns the same value.
+template <typename T>
+__device__ double group_log_mean_accept(const T* lar, int b, int e,
+                                        double* red) {
+  const int tid = threadIdx.x;
+  auto lap_of = [&](int i) -> double {
+    double v = (double)lar[i];
+    if (!__builtin_isfinite(v)) v = -__builtin_inf();
+    return fmin(v, 0.0);
+  };
+  double m = -__builtin_inf();
+  for (int i = b + tid; i < e; i += blockDim.x) m = fmax(m, lap_of(i));
+  red[tid] = m;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (tid < s) red[tid] = fmax(red[tid], red[tid + s]);
+    __syncthreads();
+  }
+  double mx = red[0];
+  if (!__builtin_isfinite(mx)) mx = 0.0;
+  __syncthreads();
+  double acc = 0.0;
+  for (int i = b + tid; i < e; i += blockDim.x) acc += exp(lap_of(i) - mx);
+  red[tid] = acc;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (tid < s) red[tid] += red[tid + s];
+    __syncthreads();
+  }
+  const double total = red[0];
+  __syncthreads();
+  return log(total) + mx - log((double)(e - b));
+}
+
 // One block per step-size group; float64 throughout
 // (dual_averaging_step_size_adaptation.py:399-475).
 template <typename T>
@@ -249,34 +285,8 @@ __global__ void __launch_bounds__(256)
   double cur = step_f64 ? step_f64[gidx] : (double)step_size[gidx];
   for (int r = 0; r < n_rows; ++r) {
     const T* lar = lar_base + (size_t)r * row_stride;
-    auto lap_of = [&](int i) -> double {
-      double v = (double)lar[i];
-      if (!__builtin_isfinite(v)) v = -__builtin_inf();
-      return fmin(v, 0.0);  // hmc_like_log_accept_prob_getter_fn
-    };
-    double m = -__builtin_inf();
-    for (int i = b + tid; i < e; i += blockDim.x) m = fmax(m, lap_of(i));
-    red[tid] = m;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-      if (tid < s) red[tid] = fmax(red[tid], red[tid + s]);
-      __syncthreads();
-    }
-    double mx = red[0];
-    if (!__builtin_isfinite(mx)) mx = 0.0;
-    __syncthreads();
-    double acc = 0.0;
-    for (int i = b + tid; i < e; i += blockDim.x) acc += exp(lap_of(i) - mx);
-    red[tid] = acc;
-    __syncthreads();
-    for (int s = 128; s > 0; s >>= 1) {
-      if (tid < s) red[tid] += red[tid + s];
-      __syncthreads();
-    }
-    const double total = red[0];
-    __syncthreads();
     // reduce_logmeanexp, then `_one_step_part`
-    const double rlap = log(total) + mx - log((double)(e - b));
+    const double rlap = group_log_mean_accept(lar, b, e, red);
     err = err + target_accept[gidx] - exp(rlap);
     const int64_t step = step0 + r;
     const double t = (double)step + 1.0;
@@ -295,6 +305,32 @@ __global__ void __launch_bounds__(256)
     step_size[gidx] = (T)cur;
     if (step_f64) step_f64[gidx] = cur;
   }
+}
+
+// SimpleStepSizeAdaptation (`simple_step_size_adaptation.py:344-438`): one
+// block per step-size element; the element grows by (1 + rate) when its
+// group's log-mean accept probability exceeds log(target), else shrinks by
+// it, while step < num_adaptation_steps.  The update is in the step-size
+// dtype, as the reference multiplies in that dtype.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    simple_step_kernel(const int32_t* step_group, const int32_t* group_start,
+                       const T* lar_base, int n_rows, int64_t row_stride,
+                       const double* target_accept, double rate,
+                       int64_t step0, int64_t num_adapt, T* step_size) {
+  __shared__ double red[256];
+  const int elem = blockIdx.x;
+  const int grp = step_group ? step_group[elem] : elem;
+  const int b = group_start[grp], e = group_start[grp + 1];
+  const T opr = T(1) + (T)rate;
+  const double log_target = (double)log((T)target_accept[elem]);
+  T cur = step_size[elem];
+  for (int r = 0; r < n_rows; ++r) {
+    const double rlap =
+        group_log_mean_accept(lar_base + (size_t)r * row_stride, b, e, red);
+    if (step0 + r < num_adapt) cur = rlap > log_target ? cur * opr : cur / opr;
+  }
+  if (threadIdx.x == 0) step_size[elem] = cur;
 }
 
 // ------------------------------------------------------------ MH kernels
@@ -583,6 +619,35 @@ int hipmc_da_update(int32_t dtype, int32_t n_groups, const int32_t* step_group,
   else
     return fail(HIPMC_E_INVALID, "bad dtype");
   return hip_check(hipGetLastError(), "da_kernel launch");
+}
+
+int hipmc_simple_step_update(int32_t dtype, int32_t n_elems,
+                             const int32_t* step_group,
+                             const int32_t* group_start, const void* lar,
+                             int32_t n_rows, int64_t row_stride,
+                             const double* target_accept_prob,
+                             double adaptation_rate, int64_t step,
+                             int64_t num_adaptation_steps, void* step_size,
+                             uint64_t stream) {
+  if (n_elems < 1 || !group_start || !lar || !target_accept_prob || !step_size)
+    return fail(HIPMC_E_INVALID,
+                "simple_step_update: NULL argument or n_elems < 1");
+  if (n_rows < 1)
+    return fail(HIPMC_E_INVALID, "simple_step_update: n_rows must be >= 1");
+  hipStream_t s = as_stream(stream);
+  if (dtype == HIPMC_F32)
+    hipLaunchKernelGGL(simple_step_kernel<float>, dim3(n_elems), dim3(256), 0,
+                       s, step_group, group_start, (const float*)lar, n_rows,
+                       row_stride, target_accept_prob, adaptation_rate, step,
+                       num_adaptation_steps, (float*)step_size);
+  else if (dtype == HIPMC_F64)
+    hipLaunchKernelGGL(simple_step_kernel<double>, dim3(n_elems), dim3(256), 0,
+                       s, step_group, group_start, (const double*)lar, n_rows,
+                       row_stride, target_accept_prob, adaptation_rate, step,
+                       num_adaptation_steps, (double*)step_size);
+  else
+    return fail(HIPMC_E_INVALID, "bad dtype");
+  return hip_check(hipGetLastError(), "simple_step_kernel launch");
 }
 
 int hipmc_mh_accept(int32_t dtype, int32_t n, int32_t stride,
