@@ -125,6 +125,54 @@ def hmc_one_step(target, step_size, num_leapfrog_steps, q, t, g, seed,
 
 
 # --------------------------------------------------------------------------
+# MALA  (`mcmc/langevin.py:427-823`), constant volatility 1
+
+MALAResults = collections.namedtuple('MALAResults', [
+    'is_accepted', 'log_accept_ratio', 'target_log_prob',
+    'grads_target_log_prob', 'proposed_state', 'proposed_target_log_prob',
+    'log_acceptance_correction', 'diffusion_drift'])
+
+
+def mala_one_step(target, step_size, q, t, g, seed, draws=None):
+  """One MetropolisAdjustedLangevinAlgorithm step for all chains, written
+  as the reference writes it: drift 0.5 * eps * grad (`_get_drift`,
+  `:637-690`, volatility 1 and no volatility gradient), Euler-Maruyama
+  proposal state + drift + sqrt(eps) * xi (`_euler_method`, `:579-635`),
+  and the normal proposal-density correction
+  log q(current | proposed) - log q(proposed | current)
+  (`_compute_log_acceptance_correction`, `:692-823`).  The proposal draw xi
+  uses the momentum draws of the proposal seed (this library's stream).
+  """
+  draws = draws or orng.PhiloxDraws()
+  c, d = q.shape
+  prop_seed, acc_seed = orng.split_seed(seed, 2)
+  xi = draws.momentum(prop_seed, c, d).astype(np.float64)
+  eps = float(step_size)
+  drift = 0.5 * eps * g
+  q1 = q + drift + np.sqrt(eps) * xi
+  t1, g1 = target.log_prob_and_grad(q1)
+  drift1 = 0.5 * eps * g1
+  vol = np.sqrt(eps)
+  with np.errstate(all='ignore'):
+    diff = q1 - q
+    e_fwd = (diff - drift) / vol
+    e_bwd = (diff + drift1) / vol
+    fwd = np.sum(np.log(vol) + 0.5 * e_fwd ** 2, -1)
+    bwd = np.sum(np.log(vol) + 0.5 * e_bwd ** 2, -1)
+    corr = fwd - bwd
+    lar = safe_sum([t1, -t, corr])
+    u = draws.mh_uniform(acc_seed, c).astype(np.float64)
+    is_acc = np.log(u) < lar
+  sel = is_acc[:, None]
+  return np.where(sel, q1, q), MALAResults(
+      is_accepted=is_acc, log_accept_ratio=lar,
+      target_log_prob=np.where(is_acc, t1, t),
+      grads_target_log_prob=np.where(sel, g1, g), proposed_state=q1,
+      proposed_target_log_prob=t1, log_acceptance_correction=corr,
+      diffusion_drift=drift1)
+
+
+# --------------------------------------------------------------------------
 # NUTS  (`mcmc/nuts.py`)
 
 
