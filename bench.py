@@ -72,6 +72,9 @@ def parse():
                            'serving_poisson200'])
   ap.add_argument('--requests', type=int, default=2000,
                   help='serving: number of requests')
+  ap.add_argument('--hbm-budget-gib', type=int, default=192,
+                  help='serving: device memory the server may hold for '
+                       'admitted requests')
   ap.add_argument('--plain-sample', type=int, default=16,
                   help='ragged GLMs: problems timed one per launch')
   ap.add_argument('--glm-waves', type=int, default=None,

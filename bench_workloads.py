@@ -389,7 +389,10 @@ def serving_poisson200(args, torch, pa):
       bij = [pa.bijectors.Identity(), pa.bijectors.Softplus(),
              pa.bijectors.Identity()]
     cfgs.append((spec, bij))
-  srv = serving.SamplerServer(iters_per_tick=16)
+  # one MI355X holds 288 GB of HBM: the server may keep 192 GiB of admitted
+  # requests' data and draws resident
+  srv = serving.SamplerServer(iters_per_tick=16,
+                              hbm_budget_bytes=args.hbm_budget_gib << 30)
   reqs = [serving.Request(spec, num_chains=16, num_adaptation_steps=500,
                           num_results=2000, seed=args.seed + k, bijectors=bij,
                           step_size=0.05, key=k)
@@ -428,7 +431,8 @@ def serving_poisson200(args, torch, pa):
                             'ragged GLMs (N~LogU[128,8192], D~U{4..128}), '
                             'half hierarchical normals (J~U{8..510}, Softplus '
                             'on tau); 16 chains, 500 adaptation + 2000 draws',
-                'requests': n_req, 'arrival_rate': rate})
+                'requests': n_req, 'arrival_rate': rate,
+                'hbm_budget_gib': args.hbm_budget_gib})
   line.update({
       'latency_s_p50_p99': [float(np.percentile(lat, 50)),
                             float(np.percentile(lat, 99))],
