@@ -179,6 +179,8 @@ def sample_ragged(specs, num_chains, num_results, bijectors=None, keys=None,
   depend only on its own size class, key and chain count, so they equal its
   draws alone.  Returns BatchResults in the order of `specs`.
   """
+  if not specs:
+    raise ValueError('need at least one problem')
   specs = [targets_lib.as_target(s) for s in specs]
   p = len(specs)
   chains = ([int(num_chains)] * p if np.isscalar(num_chains)
