@@ -22,6 +22,7 @@
 namespace hipmc {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMD = 256;       // state dimension of this path
@@ -53,8 +54,12 @@ __device__ inline void lds_dma4(const void* g, void* lds_base) {
                :: "v"(g), "s"(lds_addr(lds_base)) : "memory", "m0");
 }
 
-// X tile: 16-byte chunk (row, c) at chunk row * 32 + (c ^ (row & 7));
-// X^T tile: chunk (d, c) at d * 4 + (c ^ ((d >> 1) & 3)).
+// Chunk swizzle of X^T row d: 16 consecutive d of one chunk column land in
+// 16 distinct 16-byte bank groups, and any 8 consecutive aligned d in 8.
+__device__ inline int xt_swz(int d) { return ((d >> 1) ^ (d >> 2)) & 3; }
+
+// X tile: 16-byte chunk (row, c) at chunk row * 32 + (c ^ (row & 15));
+// X^T tile: chunk (d, c) at d * 4 + (c ^ xt_swz(d)).
 __device__ inline void stage_tile(const __bf16* X, const __bf16* Xt,
                                   const float* y, int n_obs, int n0,
                                   __bf16* xs, __bf16* xts, float* ys, int w,
@@ -63,14 +68,14 @@ __device__ inline void stage_tile(const __bf16* X, const __bf16* Xt,
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     const int q = (w * 4 + u) * 64 + l;
-    const int row = q >> 5, c = (q & 31) ^ (row & 7);
+    const int row = q >> 5, c = (q & 31) ^ (row & 15);
     lds_dma16(X + (size_t)(n0 + row) * kMD + c * 8,
               (char*)xs + (size_t)(w * 4 + u) * 1024);
   }
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
     const int q = (w * 4 + u) * 64 + l;
-    const int d = q >> 2, c = (q & 3) ^ ((d >> 1) & 3);
+    const int d = q >> 2, c = (q & 3) ^ xt_swz(d);
     lds_dma16(Xt + (size_t)d * n_obs + n0 + c * 8,
               (char*)xts + (size_t)(w * 4 + u) * 1024);
   }
@@ -83,8 +88,10 @@ __device__ inline void stage_tile(const __bf16* X, const __bf16* Xt,
 // registers) into one of two buffers while the previous tile is used, so a
 // tile's load latency hides behind the previous tile's MFMAs.  LDS layouts
 // (16-byte chunks, XOR-swizzled so that the MFMA fragment reads of 16 rows
-// hit distinct banks): X tile chunk (row, c) at row * 32 + (c ^ (row & 7)),
-// X^T tile chunk (d, c) at d * 4 + (c ^ ((d >> 1) & 3)).
+// hit distinct banks): X tile chunk (row, c) at row * 32 + (c ^ (row & 15)),
+// X^T tile chunk (d, c) at d * 4 + (c ^ xt_swz(d)).  Each lane writes its
+// four consecutive residual rows as one 8-byte store (R pitch 40 bf16 keeps
+// those stores and the 16-byte fragment reads free of bank conflicts).
 __global__ void __launch_bounds__(256, 2)
     glm_grad_mfma(const __bf16* __restrict__ X, const __bf16* __restrict__ Xt,
                   const float* __restrict__ y, int n_obs,
@@ -128,8 +135,8 @@ __global__ void __launch_bounds__(256, 2)
 #pragma unroll
     for (int ks = 0; ks < kMD / 32; ++ks) {
       const int c = 4 * ks + g;
-      const bf16x8 a0 = *(const bf16x8*)&xs[(m * 32 + (c ^ (m & 7))) * 8];
-      const bf16x8 a1 = *(const bf16x8*)&xs[((16 + m) * 32 + (c ^ (m & 7))) * 8];
+      const bf16x8 a0 = *(const bf16x8*)&xs[(m * 32 + (c ^ m)) * 8];
+      const bf16x8 a1 = *(const bf16x8*)&xs[((16 + m) * 32 + (c ^ m)) * 8];
       s0 = mfma16(a0, bh[ks], s0);
       s1 = mfma16(a1, bh[ks], s1);
       s0 = mfma16(a0, bl[ks], s0);
@@ -137,6 +144,7 @@ __global__ void __launch_bounds__(256, 2)
     }
 #pragma unroll
     for (int mt = 0; mt < 2; ++mt) {
+      bf16x4 hv, lv;
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int row = 16 * mt + 4 * g + i;
@@ -153,11 +161,11 @@ __global__ void __launch_bounds__(256, 2)
           const float w0 = 1.f - yv;
           lls += (w0 == 0.f ? 0.f : -sp_pos * w0) + (yv == 0.f ? 0.f : -sp_neg * yv);
         }
-        const __bf16 hi = (__bf16)r;
-        const __bf16 lo = (__bf16)(r - (float)hi);
-        rh_w[m * kRPitch + row] = hi;
-        rl_w[m * kRPitch + row] = lo;
+        hv[i] = (__bf16)r;
+        lv[i] = (__bf16)(r - (float)hv[i]);
       }
+      *(bf16x4*)&rh_w[m * kRPitch + 16 * mt + 4 * g] = hv;
+      *(bf16x4*)&rl_w[m * kRPitch + 16 * mt + 4 * g] = lv;
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -170,7 +178,7 @@ __global__ void __launch_bounds__(256, 2)
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const int d = 16 * (nt0 + u) + m;
-        b[u] = *(const bf16x8*)&xts[(d * 4 + (g ^ ((d >> 1) & 3))) * 8];
+        b[u] = *(const bf16x8*)&xts[(d * 4 + (g ^ xt_swz(d))) * 8];
       }
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
