@@ -27,6 +27,12 @@
  *   hipmc_simple_step_update
  *                        SimpleStepSizeAdaptation.one_step's step-size rule
  *                        (mcmc/simple_step_size_adaptation.py:344-438).
+ *   hipmc_moments_update RunningCovariance.update / RunningMean.update
+ *                        (experimental/stats/sample_stats.py:167-235,
+ *                        483-520), the state update of the streaming
+ *                        reducers (experimental/mcmc/covariance_reducer.py,
+ *                        expectations_reducer.py,
+ *                        potential_scale_reduction_reducer.py).
  *   hipmc_mh_accept      MetropolisHastings accept/reject
  *                        (mcmc/metropolis_hastings.py:204-246).
  *   hipmc_leapfrog_run   UncalibratedHamiltonianMonteCarlo.one_step proposal
@@ -347,6 +353,27 @@ int hipmc_simple_step_update(int32_t dtype, int32_t n_elems,
                              double adaptation_rate, int64_t step,
                              int64_t num_adaptation_steps, void* step_size,
                              uint64_t stream);
+
+/*
+ * One-pass (Pebay 2008) update of running moments by a chunk of chunk_n
+ * samples.  sample: dtype [chunk_n, n_batch, event_size], contiguous; mean:
+ * float64 [n_batch, event_size]; num_samples: samples already absorbed
+ * (shared by every element).  The chunk's mean m_c and residual sum
+ * S_c = sum (x - m_c)(x - m_c)' are formed in float64, then, with
+ * n' = num_samples + chunk_n and d = m_c - mean:
+ *   mean <- mean + chunk_n * d / n'
+ *   ssr  <- ssr + S_c + (num_samples * chunk_n / n') * d d'
+ * mode 0: mean only (ssr may be NULL); mode 1: ssr float64
+ * [n_batch, event_size] (per-element variance sums); mode 2: ssr float64
+ * [n_batch, event_size, event_size] (cross products within each batch
+ * member).  Replaces RunningCovariance.update / RunningMean.update
+ * (experimental/stats/sample_stats.py:167-235, 483-520), computed there in
+ * the sample dtype.
+ */
+int hipmc_moments_update(int32_t dtype, int32_t mode, int64_t n_batch,
+                         int64_t event_size, int64_t chunk_n,
+                         const void* sample, double num_samples, double* mean,
+                         double* ssr, uint64_t stream);
 
 /*
  * Metropolis-Hastings accept for n_chains chains:

@@ -2,11 +2,14 @@
 
 `probability_amd.mcmc` mirrors `tfp.mcmc` (HamiltonianMonteCarlo,
 NoUTurnSampler, TransformedTransitionKernel,
-DualAveragingStepSizeAdaptation, sample_chain, diagnostics) on top of
-libhipmc, the HIP kernels behind the C ABI in include/hipmc.h.
+DualAveragingStepSizeAdaptation, sample_chain, diagnostics) and
+`probability_amd.experimental` the streaming reducers of
+`tfp.experimental.mcmc` / `tfp.experimental.stats`, on top of libhipmc, the
+HIP kernels behind the C ABI in include/hipmc.h.
 """
 
 from probability_amd import bijectors
+from probability_amd import experimental
 from probability_amd import mcmc
 from probability_amd import random
 from probability_amd import targets

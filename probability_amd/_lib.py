@@ -28,7 +28,7 @@ EXPORTED_SYMBOLS = (
     'hipmc_seed_salt', 'hipmc_philox_host', 'hipmc_hmc_run',
     'hipmc_leapfrog_run', 'hipmc_nuts_run', 'hipmc_target_eval',
     'hipmc_da_update', 'hipmc_simple_step_update', 'hipmc_mh_accept',
-    'hipmc_bijector', 'hipmc_draws')
+    'hipmc_bijector', 'hipmc_draws', 'hipmc_moments_update')
 
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32
@@ -131,6 +131,8 @@ def lib():
       l.hipmc_mh_accept.argtypes = [_i32, _i32, _i32, _vp, _vp, _vp, _vp, _vp,
                                     _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                     _u64]
+      l.hipmc_moments_update.argtypes = [_i32, _i32, _i64, _i64, _i64, _vp,
+                                         _f64, _vp, _vp, _u64]
       l.hipmc_bijector.argtypes = [_i32, _vp, _vp, _i32, _i32, _i32, _i32,
                                    _vp, _vp, _u64]
       l.hipmc_draws.argtypes = [_i32, _i32, _vp, _i32, _i32, _i32, _i32, _i32,

@@ -441,6 +441,85 @@ int blocks_for(size_t n, int bs) { return (int)((n + bs - 1) / bs); }
 
 }  // namespace
 
+// Running moments (Pebay's one-pass update), one thread per element of the
+// mean (modes 0 and 1); x is [chunk_n, n_el] in the sample dtype.
+template <typename T>
+__global__ void moments_elem_kernel(const T* __restrict__ x, int64_t n_el,
+                                    int64_t chunk_n, double n_old,
+                                    double* __restrict__ mean,
+                                    double* __restrict__ ssr) {
+  const double cn = (double)chunk_n, n_new = n_old + cn;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_el;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    double cm = 0.0;
+    for (int64_t r = 0; r < chunk_n; ++r) cm += (double)x[r * n_el + i];
+    cm /= cn;
+    const double delta = cm - mean[i];
+    if (ssr) {
+      double cs = 0.0;  // exactly 0 for a single sample
+      for (int64_t r = 0; r < chunk_n && chunk_n > 1; ++r) {
+        const double e = (double)x[r * n_el + i] - cm;
+        cs += e * e;
+      }
+      ssr[i] = ssr[i] + cs + (n_old * cn / n_new) * delta * delta;
+    }
+    mean[i] = mean[i] + cn * delta / n_new;
+  }
+}
+
+// Cross-product sums (mode 2), one thread per (batch, i, j); reads the mean
+// before this chunk, so the mean update runs after it on the same stream.
+template <typename T>
+__global__ void moments_cross_kernel(const T* __restrict__ x, int64_t n_batch,
+                                     int64_t ev, int64_t chunk_n,
+                                     double n_old,
+                                     const double* __restrict__ mean,
+                                     double* __restrict__ ssr) {
+  const double cn = (double)chunk_n, n_new = n_old + cn;
+  const int64_t n_el = n_batch * ev, total = n_el * ev;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = t / (ev * ev), ij = t - b * ev * ev;
+    const int64_t ei = b * ev + ij / ev, ej = b * ev + ij % ev;
+    double mi = 0.0, mj = 0.0;
+    for (int64_t r = 0; r < chunk_n; ++r) {
+      mi += (double)x[r * n_el + ei];
+      mj += (double)x[r * n_el + ej];
+    }
+    mi /= cn;
+    mj /= cn;
+    double cs = 0.0;
+    for (int64_t r = 0; r < chunk_n && chunk_n > 1; ++r)
+      cs += ((double)x[r * n_el + ei] - mi) * ((double)x[r * n_el + ej] - mj);
+    const double di = mi - mean[ei], dj = mj - mean[ej];
+    ssr[t] = ssr[t] + cs + (n_old * cn / n_new) * di * dj;
+  }
+}
+
+static unsigned moments_grid(int64_t n) {
+  const int64_t blocks = (n + 255) / 256;
+  return (unsigned)(blocks < 8192 ? (blocks > 0 ? blocks : 1) : 8192);
+}
+
+template <typename T>
+static int moments_launch(int32_t mode, int64_t n_batch, int64_t ev,
+                          int64_t chunk_n, const T* x, double n_old,
+                          double* mean, double* ssr, hipStream_t s) {
+  const int64_t n_el = n_batch * ev;
+  if (mode == 2) {
+    hipLaunchKernelGGL(moments_cross_kernel<T>, dim3(moments_grid(n_el * ev)),
+                       dim3(256), 0, s, x, n_batch, ev, chunk_n, n_old, mean,
+                       ssr);
+    const int rc = hip_check(hipGetLastError(), "moments_cross_kernel launch");
+    if (rc) return rc;
+    ssr = nullptr;
+  }
+  hipLaunchKernelGGL(moments_elem_kernel<T>, dim3(moments_grid(n_el)),
+                     dim3(256), 0, s, x, n_el, chunk_n, n_old, mean,
+                     mode == 1 ? ssr : nullptr);
+  return hip_check(hipGetLastError(), "moments_elem_kernel launch");
+}
+
 extern "C" {
 
 const char* hipmc_last_error(void) { return g_err.c_str(); }
@@ -648,6 +727,29 @@ int hipmc_simple_step_update(int32_t dtype, int32_t n_elems,
   else
     return fail(HIPMC_E_INVALID, "bad dtype");
   return hip_check(hipGetLastError(), "simple_step_kernel launch");
+}
+
+int hipmc_moments_update(int32_t dtype, int32_t mode, int64_t n_batch,
+                         int64_t event_size, int64_t chunk_n,
+                         const void* sample, double num_samples, double* mean,
+                         double* ssr, uint64_t stream) {
+  if (mode < 0 || mode > 2)
+    return fail(HIPMC_E_INVALID, "moments_update: mode must be 0, 1 or 2");
+  if (n_batch < 1 || event_size < 1 || chunk_n < 1)
+    return fail(HIPMC_E_INVALID,
+                "moments_update: n_batch, event_size and chunk_n must be >= 1");
+  if (!sample || !mean || (mode > 0 && !ssr))
+    return fail(HIPMC_E_INVALID, "moments_update: NULL argument");
+  if (!(num_samples >= 0.0))
+    return fail(HIPMC_E_INVALID, "moments_update: num_samples must be >= 0");
+  hipStream_t s = as_stream(stream);
+  if (dtype == HIPMC_F32)
+    return moments_launch(mode, n_batch, event_size, chunk_n,
+                          (const float*)sample, num_samples, mean, ssr, s);
+  if (dtype == HIPMC_F64)
+    return moments_launch(mode, n_batch, event_size, chunk_n,
+                          (const double*)sample, num_samples, mean, ssr, s);
+  return fail(HIPMC_E_INVALID, "bad dtype");
 }
 
 int hipmc_mh_accept(int32_t dtype, int32_t n, int32_t stride,
