@@ -168,7 +168,8 @@ def main():
   if args.gpus != 1:
     print(f'bench.py: this build runs on one GPU; --gpus {args.gpus} is not '
           'supported (chains and posteriors shard across GPUs with no '
-          'communication; see DESIGN.md)', file=sys.stderr)
+          'communication, so N GPUs run N independent shares)',
+          file=sys.stderr)
     sys.exit(2)
   os.environ.setdefault('OMP_NUM_THREADS', '1')
   import torch
