@@ -40,7 +40,11 @@ __device__ inline f32x4 mfma16(const bf16x8& a, const bf16x8& b,
 // wave-uniform LDS base + 16 (4) * lane.  Issued from inline asm so that the
 // compiler's wait-count pass does not make every later LDS read wait for
 // them; the kernel waits for its own tile loads (vmcnt) before the barrier
-// that publishes a tile.
+// that publishes a tile.  m0 is a reserved register whose clobber the
+// compiler does not track (hipcc warns): each asm statement sets m0 and
+// issues the load that reads it, and nothing else in glm_grad_mfma uses m0
+// (no LDS-DMA builtins, no dynamically indexed register arrays), which the
+// gfx950 assembly of the kernel confirms (its only m0 writes are these).
 typedef __attribute__((address_space(3))) char lds_byte;
 __device__ inline uint32_t lds_addr(void* p) {
   return (uint32_t)(uintptr_t)(lds_byte*)p;
