@@ -52,6 +52,8 @@ def _moments_update(mode, mean, ssr, num_samples, sample, event_ndims,
   """Launches one update of (mean, ssr) in place; returns the chunk size."""
   shape = tuple(mean.shape)
   if axis is not None:
+    if sample.dim() == 0:
+      raise ValueError('a chunked update needs a sample with the chunk axis')
     axis = axis % sample.dim()
     chunk_n = sample.shape[axis]
     sample = torch.movedim(sample, axis, 0).contiguous()

@@ -232,6 +232,10 @@ def _run_fused(plan, num_results, current_state, num_burnin_steps, thin,
   else:
     step0 = base_res.step_size
   step = _runner.step_tensor(step0, dt, dev)
+  # traced step sizes keep the shape the step loop reports (a scalar step
+  # size traces as [num_results]); list step sizes stay flat
+  step_shape = (None if isinstance(step0, (list, tuple))
+                else tuple(torch.as_tensor(step0).shape))
   n_total = (num_burnin_steps + 1 + (num_results - 1) * (1 + thin)
              if num_results else num_burnin_steps)
   table = spec.table(spec.bijectors_for_eval())
@@ -364,8 +368,11 @@ def _run_fused(plan, num_results, current_state, num_burnin_steps, thin,
         v = step_hist[rec_idx]
       elif name == 'target_log_prob':
         raise NotImplementedError('target_log_prob is traced by the step loop')
-      vals.append(v.reshape((num_results,) + cs + tuple(v.shape[2:]))
-                  if name != 'step_size' else v)
+      if name != 'step_size':
+        v = v.reshape((num_results,) + cs + tuple(v.shape[2:]))
+      elif step_shape is not None:
+        v = v.reshape((num_results,) + step_shape)
+      vals.append(v)
     trace = vals[0] if len(vals) == 1 else tuple(vals)
   if return_final:
     final = _final_results(plan, pkr, spec, state, tlp, grad, step, cs,

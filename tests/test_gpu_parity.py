@@ -105,8 +105,8 @@ def _case(kind, dtype_np, seed=0, c=64):
     codes = np.zeros(7, np.int32)
     codes[6] = 2  # Exp on sigma
     return o, mk, q, codes
-  if kind == 'hier':
-    j = 70
+  if kind in ('hier', 'hier256'):
+    j = 70 if kind == 'hier' else 256  # 256: the headline's groups
     y = cast(4 + 3 * r.normal(size=j) + r.uniform(5, 20, j) * r.normal(size=j))
     s = cast(r.uniform(5, 20, j))
     o = otg.HierarchicalNormal(y, s)
@@ -274,7 +274,8 @@ def _nuts_one(kind, tdt, ndt, seed=0, c=64, eps=None, max_depth=10):
   o, mk, q, codes = _case(kind, ndt, seed=seed, c=c)
   spec = mk(tdt)
   ot = _otarget(o, codes)
-  eps = eps or {'logistic': 0.08, 'linear': 0.05, 'hier': 0.3, 'diag': 0.5,
+  eps = eps or {'logistic': 0.08, 'linear': 0.05, 'hier': 0.3,
+                'hier256': 0.3, 'diag': 0.5,
                 'funnel': 0.9, 'logistic_split': 0.04,
                 'linear_split': 0.02, 'logistic_wide': 0.02,
                 'logistic_d150': 0.05, 'linear_wide': 0.01,
@@ -296,10 +297,11 @@ def _nuts_one(kind, tdt, ndt, seed=0, c=64, eps=None, max_depth=10):
 
 
 @pytest.mark.parametrize('tdt,ndt,tol', DTYPES)
-@pytest.mark.parametrize('kind', ['hier', 'logistic', 'linear', 'diag',
-                                  'funnel', 'logistic_split', 'linear_split',
-                                  'logistic_wide', 'logistic_d150',
-                                  'linear_wide', 'logistic_tall'])
+@pytest.mark.parametrize('kind', ['hier', 'hier256', 'logistic', 'linear',
+                                  'diag', 'funnel', 'logistic_split',
+                                  'linear_split', 'logistic_wide',
+                                  'logistic_d150', 'linear_wide',
+                                  'logistic_tall'])
 def test_nuts_one_step_matches_oracle(kind, tdt, ndt, tol):
   for seed in range(3):
     spec, nxt, res, oq, ores = _nuts_one(kind, tdt, ndt, seed=seed)
@@ -417,10 +419,27 @@ def test_fused_sample_chain_equals_step_loop(tdt, ndt, tol):
     np.testing.assert_array_equal(_np(pa), _np(pb))
   np.testing.assert_array_equal(_np(a.trace[0]), _np(b.trace[0]))
   np.testing.assert_array_equal(_np(a.trace[1]), _np(b.trace[1]))
-  np.testing.assert_allclose(_np(a.trace[2]).reshape(-1),
-                             _np(b.trace[2]).reshape(-1), rtol=0)
+  np.testing.assert_array_equal(_np(a.trace[2]), _np(b.trace[2]))
+  assert tuple(a.trace[2].shape) == (12,)  # scalar step size: [num_results]
   np.testing.assert_array_equal(_np(a.final_kernel_results.new_step_size),
                                 _np(b.final_kernel_results.new_step_size))
+
+
+def test_headline_shape_fused_equals_step_loop():
+  """The headline's shape (J = 256 groups, 1024 chains, float32, NUTS + DA
+  + TTK): the fused device loop equals the host step loop bit for bit."""
+  spec, da, init, _, _ = _hier_kernel(torch.float32, np.float32, j=256,
+                                      c=1024, adapt=3)
+  tf = pmcmc.TraceFields('log_accept_ratio', 'leapfrogs_taken', 'step_size')
+  kw = dict(num_burnin_steps=3, trace_fn=tf, seed=9,
+            return_final_kernel_results=True)
+  a = pmcmc.sample_chain(3, init, kernel=da, **kw)
+  b = pmcmc.sample_chain(3, init, kernel=da, experimental_fused=False, **kw)
+  for pa, pb in zip(a.all_states, b.all_states):
+    assert tuple(pa.shape[1:2]) == (1024,)
+    np.testing.assert_array_equal(_np(pa), _np(pb))
+  for k in range(3):
+    np.testing.assert_array_equal(_np(a.trace[k]), _np(b.trace[k]))
 
 
 @pytest.mark.parametrize('tdt,ndt,tol', DTYPES)
