@@ -64,3 +64,29 @@ def test_mfma_hmc_step_matches_oracle(n, c):
   sel = got_acc == ores.is_accepted
   np.testing.assert_allclose(nxt.cpu().numpy()[sel], oq[sel], rtol=2e-3,
                              atol=2e-3)
+
+
+def test_mfma_gradient_at_full_size_matches_float64_torch():
+  """The C2 shape (65536 x 256 BF16 X, 4096 chains, eight row splits): the
+  MFMA gradient and log-prob against a float64 torch evaluation of the same
+  BF16-rounded X (same tolerance as the small case)."""
+  n, c = 65536, 4096
+  g = torch.Generator(device='cuda').manual_seed(3)
+  x = torch.randn(n, 256, device='cuda', generator=g) / 16
+  beta = torch.randn(256, device='cuda', generator=g)
+  y = (torch.rand(n, device='cuda', generator=g) <
+       torch.sigmoid(x @ beta)).float()
+  spec = ptg.LogisticRegressionBF16(x, y)
+  q = torch.randn(c, 256, device='cuda', generator=g) * 0.3
+  tlp, grad = spec.log_prob_and_grad_flat(q)
+  xr, y64, q64 = spec.x.double(), y.double(), q.double()
+  s = q64 @ xr.t()                                      # [C, N] logits
+  ll = (y64 * s - torch.nn.functional.softplus(s)).sum(1)
+  lp = ll - 0.5 * (q64 ** 2).sum(1) - 256 * 0.5 * np.log(2 * np.pi)
+  rg = (y64 - torch.sigmoid(s)) @ xr - q64
+  del s
+  np.testing.assert_allclose(tlp.double().cpu().numpy(), lp.cpu().numpy(),
+                             rtol=1e-4)
+  rg = rg.cpu().numpy()
+  np.testing.assert_allclose(grad.double().cpu().numpy(), rg, rtol=2e-3,
+                             atol=2e-3 * np.abs(rg).max())
