@@ -69,7 +69,7 @@ def parse():
   ap.add_argument('--workload', default='hier_normal_j256',
                   choices=['hier_normal_j256', 'logistic_1024x25',
                            'logistic_65536x256', 'ragged_glms_512',
-                           'serving_poisson200'])
+                           'serving_poisson200', 'reducers_1024x258'])
   ap.add_argument('--requests', type=int, default=2000,
                   help='serving: number of requests')
   ap.add_argument('--hbm-budget-gib', type=int, default=192,

@@ -12,6 +12,11 @@
                       each, NUTS + per-problem dual averaging (target 0.8),
                       500 adaptation + 1000 draws, all problems in shared
                       launches; beside it the same problems one per launch.
+* reducers_1024x258   the streaming reducers at the headline's state shape
+                      (1024 chains x 258 parameters, float32 draws):
+                      running mean, per-element variance, per-chain
+                      covariance (event_ndims = 1) and streaming R-hat, one
+                      update per draw, 64 draws per step.
 * serving_poisson200  the continuous-batching server under Poisson arrivals
                       (200 req/s); half the requests are GLMs from the ragged
                       distribution, half hierarchical normals; 16 chains,
@@ -445,9 +450,74 @@ def serving_poisson200(args, torch, pa):
   return line
 
 
+# ------------------------------------------------------------ reducers
+def reducers_1024x258(args, torch, pa):
+  """Streaming reducer updates (hipmc_moments_update) per draw at the
+  headline's shape.  Algorithmic HBM bytes per update: the draw (C D 4 B)
+  plus reading and writing the float64 sums: C D 16 B for the mean, C D 16 B
+  more for per-element variances, C D^2 16 B more for per-chain
+  covariances."""
+  from probability_amd.experimental import mcmc as emcmc
+  c, d, draws = 1024, 258, 64
+  g = torch.Generator(device='cuda').manual_seed(args.seed)
+  xs = torch.randn(draws, c, d, device='cuda', generator=g)
+  kinds = {
+      'mean': (emcmc.ExpectationsReducer(), c * d * (4 + 16)),
+      'variance': (emcmc.VarianceReducer(), c * d * (4 + 32)),
+      'covariance': (emcmc.CovarianceReducer(event_ndims=1),
+                     c * d * (4 + 16) + c * d * d * 16),
+      'rhat': (emcmc.PotentialScaleReductionReducer(), c * d * (4 + 32)),
+  }
+  out = {}
+  for name, (red, nbytes) in kinds.items():
+
+    def step(k, red=red):
+      s = red.initialize(xs[0])
+      for t in range(draws):
+        s = red.one_step(xs[t], s)
+      return s
+    _, ms = _timed(torch, step, args.steps, args.warmup)
+    # kernel time alone, HIP events on the stream the updates run on
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(
+        enable_timing=True)
+    st = red.initialize(xs[0])
+    e0.record()
+    for t in range(draws):
+      st = red.one_step(xs[t], st)
+    e1.record()
+    torch.cuda.synchronize()
+    kms = e0.elapsed_time(e1) / draws
+    out[name] = {'ms_per_update': ms / draws, 'event_ms_per_update': kms,
+                 'bytes_per_update': nbytes,
+                 'achieved_GBps': nbytes / (kms * 1e-3) / 1e9}
+    _say(f'reducers: {name} {kms * 1e3:.1f} us per update, '
+         f'{out[name]["achieved_GBps"]:.0f} GB/s')
+  cov = out['covariance']
+  ms = sum(v['ms_per_update'] for v in out.values()) * draws
+  line = _base('draws reduced per second (mean + variance + covariance + '
+               'R-hat reducers, 1024 chains x 258)', draws / (ms / 1e3),
+               'draws/s', args, ms, 'f64 sums of f32 draws',
+               {'workload': 'reducers_1024x258: ExpectationsReducer, '
+                            'VarianceReducer, CovarianceReducer(event_ndims=1) '
+                            'and PotentialScaleReductionReducer, one update '
+                            'per draw, 64 draws per step', 'chains': c,
+                'dim': d, 'draws_per_step': draws})
+  line.update({
+      'reducers': out,
+      'roofline': {'bound': 'hbm', 'achieved': cov['achieved_GBps'],
+                   'peak': HBM_PEAK_GBS, 'unit': 'GB/s',
+                   'frac': cov['achieved_GBps'] / HBM_PEAK_GBS,
+                   'traffic': None,
+                   'kernel': 'moments_cross_kernel + moments_elem_kernel '
+                             '(covariance update)',
+                   'bytes_per_update': cov['bytes_per_update']}})
+  return line
+
+
 def run(args, torch, pa):
   fn = {'logistic_1024x25': logistic_1024x25,
         'logistic_65536x256': logistic_65536x256,
         'ragged_glms_512': ragged_glms_512,
-        'serving_poisson200': serving_poisson200}[args.workload]
+        'serving_poisson200': serving_poisson200,
+        'reducers_1024x258': reducers_1024x258}[args.workload]
   return fn(args, torch, pa)

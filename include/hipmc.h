@@ -357,23 +357,26 @@ int hipmc_simple_step_update(int32_t dtype, int32_t n_elems,
 /*
  * One-pass (Pebay 2008) update of running moments by a chunk of chunk_n
  * samples.  sample: dtype [chunk_n, n_batch, event_size], contiguous; mean:
- * float64 [n_batch, event_size]; num_samples: samples already absorbed
- * (shared by every element).  The chunk's mean m_c and residual sum
+ * float64 [n_batch, event_size], read from mean_in and written to mean_out
+ * (ssr likewise; out may equal in for an in-place update, other overlaps
+ * are not allowed); num_samples: samples already absorbed (shared by every
+ * element).  The chunk's mean m_c and residual sum
  * S_c = sum (x - m_c)(x - m_c)' are formed in float64, then, with
  * n' = num_samples + chunk_n and d = m_c - mean:
  *   mean <- mean + chunk_n * d / n'
  *   ssr  <- ssr + S_c + (num_samples * chunk_n / n') * d d'
- * mode 0: mean only (ssr may be NULL); mode 1: ssr float64
+ * mode 0: mean only (ssr_in / ssr_out may be NULL); mode 1: ssr float64
  * [n_batch, event_size] (per-element variance sums); mode 2: ssr float64
  * [n_batch, event_size, event_size] (cross products within each batch
- * member).  Replaces RunningCovariance.update / RunningMean.update
+ * member; event_size <= 46340 in this mode).  Replaces RunningCovariance.update / RunningMean.update
  * (experimental/stats/sample_stats.py:167-235, 483-520), computed there in
  * the sample dtype.
  */
 int hipmc_moments_update(int32_t dtype, int32_t mode, int64_t n_batch,
                          int64_t event_size, int64_t chunk_n,
-                         const void* sample, double num_samples, double* mean,
-                         double* ssr, uint64_t stream);
+                         const void* sample, double num_samples,
+                         const double* mean_in, const double* ssr_in,
+                         double* mean_out, double* ssr_out, uint64_t stream);
 
 /*
  * Metropolis-Hastings accept for n_chains chains:

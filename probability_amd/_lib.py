@@ -132,7 +132,7 @@ def lib():
                                     _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
                                     _u64]
       l.hipmc_moments_update.argtypes = [_i32, _i32, _i64, _i64, _i64, _vp,
-                                         _f64, _vp, _vp, _u64]
+                                         _f64, _vp, _vp, _vp, _vp, _u64]
       l.hipmc_bijector.argtypes = [_i32, _vp, _vp, _i32, _i32, _i32, _i32,
                                    _vp, _vp, _u64]
       l.hipmc_draws.argtypes = [_i32, _i32, _vp, _i32, _i32, _i32, _i32, _i32,

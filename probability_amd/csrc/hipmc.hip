@@ -442,57 +442,108 @@ int blocks_for(size_t n, int bs) { return (int)((n + bs - 1) / bs); }
 }  // namespace
 
 // Running moments (Pebay's one-pass update), one thread per element of the
-// mean (modes 0 and 1); x is [chunk_n, n_el] in the sample dtype.
+// mean (modes 0 and 1); x is [chunk_n, n_el] in the sample dtype.  Reads the
+// sums before the chunk (mean_in, ssr_in) and writes the updated ones
+// (mean_out, ssr_out; the same buffers for an in-place update).
 template <typename T>
 __global__ void moments_elem_kernel(const T* __restrict__ x, int64_t n_el,
                                     int64_t chunk_n, double n_old,
-                                    double* __restrict__ mean,
-                                    double* __restrict__ ssr) {
+                                    const double* mean_in,
+                                    const double* ssr_in, double* mean_out,
+                                    double* ssr_out) {
   const double cn = (double)chunk_n, n_new = n_old + cn;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n_el;
        i += (int64_t)gridDim.x * blockDim.x) {
     double cm = 0.0;
     for (int64_t r = 0; r < chunk_n; ++r) cm += (double)x[r * n_el + i];
     cm /= cn;
-    const double delta = cm - mean[i];
-    if (ssr) {
+    const double m = mean_in[i];
+    const double delta = cm - m;
+    if (ssr_out) {
       double cs = 0.0;  // exactly 0 for a single sample
       for (int64_t r = 0; r < chunk_n && chunk_n > 1; ++r) {
         const double e = (double)x[r * n_el + i] - cm;
         cs += e * e;
       }
-      ssr[i] = ssr[i] + cs + (n_old * cn / n_new) * delta * delta;
+      ssr_out[i] = ssr_in[i] + cs + (n_old * cn / n_new) * delta * delta;
     }
-    mean[i] = mean[i] + cn * delta / n_new;
+    mean_out[i] = m + cn * delta / n_new;
   }
 }
 
-// Cross-product sums (mode 2), one thread per (batch, i, j); reads the mean
+// Cross-product sums (mode 2): blockIdx.y strides over batch members, and
+// the threads of the blockIdx.x range stride over the member's ev x ev sums
+// (contiguous, so reads and writes coalesce), splitting the flat index with
+// 32-bit arithmetic (the host keeps ev * ev below 2^31).  Reads the mean
 // before this chunk, so the mean update runs after it on the same stream.
 template <typename T>
 __global__ void moments_cross_kernel(const T* __restrict__ x, int64_t n_batch,
                                      int64_t ev, int64_t chunk_n,
                                      double n_old,
-                                     const double* __restrict__ mean,
-                                     double* __restrict__ ssr) {
+                                     const double* __restrict__ mean_in,
+                                     const double* ssr_in, double* ssr_out) {
   const double cn = (double)chunk_n, n_new = n_old + cn;
-  const int64_t n_el = n_batch * ev, total = n_el * ev;
-  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
-       t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t b = t / (ev * ev), ij = t - b * ev * ev;
-    const int64_t ei = b * ev + ij / ev, ej = b * ev + ij % ev;
-    double mi = 0.0, mj = 0.0;
-    for (int64_t r = 0; r < chunk_n; ++r) {
-      mi += (double)x[r * n_el + ei];
-      mj += (double)x[r * n_el + ej];
+  const double adj = n_old * cn / n_new;
+  const int64_t n_el = n_batch * ev;
+  const uint32_t e32 = (uint32_t)ev, ev2 = e32 * e32;
+  const uint32_t stride = gridDim.x * blockDim.x;
+  if ((e32 & 1u) == 0) {
+    // even ev: each thread takes two neighbours (j, j + 1) of one row, with
+    // 16-byte accesses to the sums and the mean (j is even, so j + 1 < ev)
+    const uint32_t n_pairs = ev2 >> 1;
+    for (int64_t b = blockIdx.y; b < n_batch; b += gridDim.y) {
+      const size_t base = (size_t)b * ev2;
+      for (uint32_t pi = blockIdx.x * blockDim.x + threadIdx.x; pi < n_pairs;
+           pi += stride) {
+        const uint32_t ij = pi << 1;
+        const uint32_t i = ij / e32, j = ij - i * e32;
+        const int64_t ei = b * ev + i, ej = b * ev + j;
+        double mi = 0.0, m0 = 0.0, m1 = 0.0;
+        for (int64_t r = 0; r < chunk_n; ++r) {
+          mi += (double)x[r * n_el + ei];
+          m0 += (double)x[r * n_el + ej];
+          m1 += (double)x[r * n_el + ej + 1];
+        }
+        mi /= cn;
+        m0 /= cn;
+        m1 /= cn;
+        double c0 = 0.0, c1 = 0.0;  // exactly 0 for a single sample
+        for (int64_t r = 0; r < chunk_n && chunk_n > 1; ++r) {
+          const double e = (double)x[r * n_el + ei] - mi;
+          c0 += e * ((double)x[r * n_el + ej] - m0);
+          c1 += e * ((double)x[r * n_el + ej + 1] - m1);
+        }
+        const double di = adj * (mi - mean_in[ei]);
+        const double2 mj = *(const double2*)(mean_in + ej);
+        const double2 old = *(const double2*)(ssr_in + base + ij);
+        double2 nv;
+        nv.x = old.x + c0 + di * (m0 - mj.x);
+        nv.y = old.y + c1 + di * (m1 - mj.y);
+        *(double2*)(ssr_out + base + ij) = nv;
+      }
     }
-    mi /= cn;
-    mj /= cn;
-    double cs = 0.0;
-    for (int64_t r = 0; r < chunk_n && chunk_n > 1; ++r)
-      cs += ((double)x[r * n_el + ei] - mi) * ((double)x[r * n_el + ej] - mj);
-    const double di = mi - mean[ei], dj = mj - mean[ej];
-    ssr[t] = ssr[t] + cs + (n_old * cn / n_new) * di * dj;
+    return;
+  }
+  for (int64_t b = blockIdx.y; b < n_batch; b += gridDim.y) {
+    const size_t base = (size_t)b * ev2;
+    for (uint32_t ij = blockIdx.x * blockDim.x + threadIdx.x; ij < ev2;
+         ij += stride) {
+      const uint32_t i = ij / e32, j = ij - i * e32;
+      const int64_t ei = b * ev + i, ej = b * ev + j;
+      double mi = 0.0, mj = 0.0;
+      for (int64_t r = 0; r < chunk_n; ++r) {
+        mi += (double)x[r * n_el + ei];
+        mj += (double)x[r * n_el + ej];
+      }
+      mi /= cn;
+      mj /= cn;
+      double cs = 0.0;  // exactly 0 for a single sample
+      for (int64_t r = 0; r < chunk_n && chunk_n > 1; ++r)
+        cs += ((double)x[r * n_el + ei] - mi) *
+              ((double)x[r * n_el + ej] - mj);
+      ssr_out[base + ij] = ssr_in[base + ij] + cs +
+                           adj * (mi - mean_in[ei]) * (mj - mean_in[ej]);
+    }
   }
 }
 
@@ -504,19 +555,28 @@ static unsigned moments_grid(int64_t n) {
 template <typename T>
 static int moments_launch(int32_t mode, int64_t n_batch, int64_t ev,
                           int64_t chunk_n, const T* x, double n_old,
-                          double* mean, double* ssr, hipStream_t s) {
+                          const double* mean_in, const double* ssr_in,
+                          double* mean_out, double* ssr_out, hipStream_t s) {
   const int64_t n_el = n_batch * ev;
   if (mode == 2) {
-    hipLaunchKernelGGL(moments_cross_kernel<T>, dim3(moments_grid(n_el * ev)),
-                       dim3(256), 0, s, x, n_batch, ev, chunk_n, n_old, mean,
-                       ssr);
+    if (ev > 46340)  // ev * ev must stay below 2^31
+      return fail(HIPMC_E_UNSUPPORTED,
+                  "moments_update: cross products of more than 46340 "
+                  "event elements are not built");
+    const int64_t ev2 = ev * ev;
+    const unsigned gx = (unsigned)((ev2 + 255) / 256 < 64 ? (ev2 + 255) / 256 : 64);
+    const unsigned gy = (unsigned)(n_batch < 65535 ? n_batch : 65535);
+    hipLaunchKernelGGL(moments_cross_kernel<T>, dim3(gx, gy), dim3(256), 0, s,
+                       x, n_batch, ev, chunk_n, n_old, mean_in, ssr_in,
+                       ssr_out);
     const int rc = hip_check(hipGetLastError(), "moments_cross_kernel launch");
     if (rc) return rc;
-    ssr = nullptr;
   }
+  const bool elem_ssr = mode == 1;
   hipLaunchKernelGGL(moments_elem_kernel<T>, dim3(moments_grid(n_el)),
-                     dim3(256), 0, s, x, n_el, chunk_n, n_old, mean,
-                     mode == 1 ? ssr : nullptr);
+                     dim3(256), 0, s, x, n_el, chunk_n, n_old, mean_in,
+                     elem_ssr ? ssr_in : nullptr, mean_out,
+                     elem_ssr ? ssr_out : nullptr);
   return hip_check(hipGetLastError(), "moments_elem_kernel launch");
 }
 
@@ -731,24 +791,27 @@ int hipmc_simple_step_update(int32_t dtype, int32_t n_elems,
 
 int hipmc_moments_update(int32_t dtype, int32_t mode, int64_t n_batch,
                          int64_t event_size, int64_t chunk_n,
-                         const void* sample, double num_samples, double* mean,
-                         double* ssr, uint64_t stream) {
+                         const void* sample, double num_samples,
+                         const double* mean_in, const double* ssr_in,
+                         double* mean_out, double* ssr_out, uint64_t stream) {
   if (mode < 0 || mode > 2)
     return fail(HIPMC_E_INVALID, "moments_update: mode must be 0, 1 or 2");
   if (n_batch < 1 || event_size < 1 || chunk_n < 1)
     return fail(HIPMC_E_INVALID,
                 "moments_update: n_batch, event_size and chunk_n must be >= 1");
-  if (!sample || !mean || (mode > 0 && !ssr))
+  if (!sample || !mean_in || !mean_out || (mode > 0 && (!ssr_in || !ssr_out)))
     return fail(HIPMC_E_INVALID, "moments_update: NULL argument");
   if (!(num_samples >= 0.0))
     return fail(HIPMC_E_INVALID, "moments_update: num_samples must be >= 0");
   hipStream_t s = as_stream(stream);
   if (dtype == HIPMC_F32)
     return moments_launch(mode, n_batch, event_size, chunk_n,
-                          (const float*)sample, num_samples, mean, ssr, s);
+                          (const float*)sample, num_samples, mean_in, ssr_in,
+                          mean_out, ssr_out, s);
   if (dtype == HIPMC_F64)
     return moments_launch(mode, n_batch, event_size, chunk_n,
-                          (const double*)sample, num_samples, mean, ssr, s);
+                          (const double*)sample, num_samples, mean_in, ssr_in,
+                          mean_out, ssr_out, s);
   return fail(HIPMC_E_INVALID, "bad dtype");
 }
 

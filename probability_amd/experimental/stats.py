@@ -8,7 +8,8 @@ one `hipmc_moments_update` launch (Pebay's one-pass chunk merge,
 `sample_stats.py:167-235`) on the current stream; the sums are held in
 float64 on the GPU whatever the sample dtype (the reference accumulates in
 the sample dtype), and the finalizers return the sample dtype.  Updates
-return a new object; the old one stays valid.
+return a new object whose sums the kernel writes into fresh buffers while
+reading the old ones (no copy), so the old object stays valid.
 """
 
 import math
@@ -49,7 +50,8 @@ def _as_sample(x, dtype, device):
 
 def _moments_update(mode, mean, ssr, num_samples, sample, event_ndims,
                     axis):
-  """Launches one update of (mean, ssr) in place; returns the chunk size."""
+  """Launches one update reading (mean, ssr) and writing fresh buffers;
+  returns (chunk size, new mean, new ssr)."""
   shape = tuple(mean.shape)
   if axis is not None:
     if sample.dim() == 0:
@@ -65,19 +67,22 @@ def _moments_update(mode, mean, ssr, num_samples, sample, event_ndims,
     raise ValueError(f'sample shape {body} does not match the running '
                      f'statistic\'s shape {shape}')
   if chunk_n < 1:
-    return 0
+    return 0, mean, ssr
   if mode == _MODE_COV:  # [n_batch, ev] samples, [n_batch, ev, ev] sums
     ev = math.prod(shape[len(shape) - event_ndims:])
     n_batch = math.prod(shape[:len(shape) - event_ndims])
   else:  # element-wise
     ev, n_batch = math.prod(shape), 1
   if ev == 0 or n_batch == 0:
-    return chunk_n
+    return chunk_n, mean.clone(), None if ssr is None else ssr.clone()
+  new_mean = torch.empty_like(mean)
+  new_ssr = None if ssr is None else torch.empty_like(ssr)
   _lib.check(_lib.lib().hipmc_moments_update(
       _lib.dtype_code(sample.dtype), mode, n_batch, ev, chunk_n,
       _lib.ptr(sample), float(num_samples), _lib.ptr(mean), _lib.ptr(ssr),
+      _lib.ptr(new_mean), _lib.ptr(new_ssr),
       _lib.stream_handle(mean.device)))
-  return chunk_n
+  return chunk_n, new_mean, new_ssr
 
 
 class RunningMean:
@@ -106,8 +111,8 @@ class RunningMean:
   def update(self, new_sample, axis=None):
     """`sample_stats.py:483-520`."""
     x = _as_sample(new_sample, self._dtype, self._mean.device)
-    mean = self._mean.clone()
-    cn = _moments_update(_MODE_MEAN, mean, None, self.num_samples, x, 0, axis)
+    cn, mean, _ = _moments_update(_MODE_MEAN, self._mean, None,
+                                  self.num_samples, x, 0, axis)
     return RunningMean(self.num_samples + cn, mean, self._dtype)
 
   def __repr__(self):
@@ -169,9 +174,9 @@ class RunningCovariance:
   def update(self, new_sample, axis=None):
     """`sample_stats.py:167-235`."""
     x = _as_sample(new_sample, self._dtype, self._mean.device)
-    mean, ssr = self._mean.clone(), self._ssr.clone()
-    cn = _moments_update(self._mode(), mean, ssr, self.num_samples, x,
-                         self.event_ndims, axis)
+    cn, mean, ssr = _moments_update(self._mode(), self._mean, self._ssr,
+                                    self.num_samples, x, self.event_ndims,
+                                    axis)
     return type(self)(self.num_samples + cn, mean, ssr, self.event_ndims,
                       self._dtype)
 
