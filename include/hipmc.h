@@ -368,7 +368,8 @@ int hipmc_simple_step_update(int32_t dtype, int32_t n_elems,
  * mode 0: mean only (ssr_in / ssr_out may be NULL); mode 1: ssr float64
  * [n_batch, event_size] (per-element variance sums); mode 2: ssr float64
  * [n_batch, event_size, event_size] (cross products within each batch
- * member; event_size <= 46340 in this mode).  Replaces RunningCovariance.update / RunningMean.update
+ * member; event_size <= 46340 in this mode).  Replaces
+ * RunningCovariance.update / RunningMean.update
  * (experimental/stats/sample_stats.py:167-235, 483-520), computed there in
  * the sample dtype.
  */
