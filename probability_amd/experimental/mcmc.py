@@ -15,7 +15,10 @@ reducer and state part per step), so a long run needs no `[R, C, D]` trace:
   (`potential_scale_reduction_reducer.py:36-150`): streaming R-hat with the
   leading `independent_chain_ndims` dimensions as chains.
 
-`sample_fold` (`sample_fold.py:35-170`) runs a kernel for `num_steps`
+`step_kernel` (`step.py:32-110`) runs a kernel for a number of steps and
+`SampleDiscardingKernel` (`sample_discarding_kernel.py:41-170`) keeps one
+draw per call, skipping burn-in and thinned draws.  `sample_fold`
+(`sample_fold.py:35-170`) runs a kernel for `num_steps`
 results, with burn-in and thinning as `SampleDiscardingKernel`
 (`sample_discarding_kernel.py:81-130`: the first result skips
 `num_burnin_steps + num_steps_between_results` draws, later ones
@@ -30,8 +33,10 @@ from probability_amd.mcmc import kernel as kernel_base
 from probability_amd.mcmc.internal import util as mcmc_util
 
 __all__ = ['CovarianceReducer', 'ExpectationsReducer',
-           'PotentialScaleReductionReducer', 'Reducer', 'VarianceReducer',
-           'WithReductions', 'WithReductionsKernelResults', 'sample_fold']
+           'PotentialScaleReductionReducer', 'Reducer',
+           'SampleDiscardingKernel', 'SampleDiscardingKernelResults',
+           'VarianceReducer', 'WithReductions', 'WithReductionsKernelResults',
+           'sample_fold', 'step_kernel']
 
 
 # ------------------------------------------------------------- structures
@@ -236,6 +241,75 @@ class WithReductions(kernel_base.TransitionKernel):
       previous_reducer_state = _reducer_map(
           lambda r: r.initialize(init_state, adjusted), self.reducer)
     return WithReductionsKernelResults(previous_reducer_state, inner_results)
+
+
+# ------------------------------------------------ step_kernel / discarding
+def step_kernel(num_steps, current_state, previous_kernel_results=None,
+                kernel=None, return_final_kernel_results=False,
+                parallel_iterations=10, seed=None, name=None):
+  """Runs `kernel` `num_steps` times; returns the last state (and the last
+  kernel results).  Step k uses seed fold_in(sanitize_seed(seed), k); with
+  seed None the kernel draws its own."""
+  del parallel_iterations, name
+  if kernel is None:
+    raise ValueError('step_kernel needs a kernel')
+  if previous_kernel_results is None:
+    previous_kernel_results = kernel.bootstrap_results(current_state)
+  base = (samplers.sanitize_seed(seed, salt='experimental.mcmc.step_kernel')
+          if seed is not None else None)
+  state, kr = current_state, previous_kernel_results
+  for k in range(int(num_steps)):
+    state, kr = kernel.one_step(
+        state, kr, seed=None if base is None else samplers.fold_in(base, k))
+  if return_final_kernel_results:
+    return state, kr
+  return state
+
+
+class SampleDiscardingKernelResults(
+    mcmc_util.PrettyNamedTupleMixin,
+    collections.namedtuple('SampleDiscardingKernelResults',
+                           'call_counter, inner_results')):
+  """`sample_discarding_kernel.py:33-38`."""
+  __slots__ = ()
+
+
+class SampleDiscardingKernel(kernel_base.TransitionKernel):
+  """Each call runs the inner kernel `num_steps_between_results + 1`
+  times (plus `num_burnin_steps` on the first call) and returns the last
+  state (`sample_discarding_kernel.py:81-130`)."""
+
+  def __init__(self, inner_kernel, num_burnin_steps=0,
+               num_steps_between_results=0, name=None):
+    if int(num_burnin_steps) < 0 or int(num_steps_between_results) < 0:
+      raise ValueError('num_burnin_steps and num_steps_between_results '
+                       'must be >= 0')
+    self.inner_kernel = inner_kernel
+    self.num_burnin_steps = int(num_burnin_steps)
+    self.num_steps_between_results = int(num_steps_between_results)
+    self.name = name or 'sample_discarding_kernel'
+
+  @property
+  def is_calibrated(self):
+    return self.inner_kernel.is_calibrated
+
+  def _num_samples_to_skip(self, call_counter):
+    return (self.num_steps_between_results +
+            (self.num_burnin_steps if int(call_counter) == 0 else 0))
+
+  def one_step(self, current_state, previous_kernel_results, seed=None):
+    state, inner = step_kernel(
+        self._num_samples_to_skip(previous_kernel_results.call_counter) + 1,
+        current_state, previous_kernel_results.inner_results,
+        kernel=self.inner_kernel, return_final_kernel_results=True,
+        seed=seed)
+    return state, SampleDiscardingKernelResults(
+        int(previous_kernel_results.call_counter) + 1, inner)
+
+  def bootstrap_results(self, init_state, inner_results=None):
+    if inner_results is None:
+      inner_results = self.inner_kernel.bootstrap_results(init_state)
+    return SampleDiscardingKernelResults(0, inner_results)
 
 
 # -------------------------------------------------------------- sample_fold

@@ -210,3 +210,14 @@ def test_sample_fold_burnin_and_thinning():
       2, q0, kernel=kern, reducer=emcmc.VarianceReducer(), seed=3,
       return_final_reducer_states=True)
   assert states.cov_state.num_samples == 2
+
+
+def test_sample_discarding_kernel_over_hmc():
+  kern, q0 = _hmc(c=8, d=3)
+  sdk = emcmc.SampleDiscardingKernel(kern, num_burnin_steps=2,
+                                     num_steps_between_results=1)
+  kr = sdk.bootstrap_results(q0)
+  q, kr = sdk.one_step(q0, kr, seed=11)
+  # the same inner steps by hand: 2 burn-in + 1 thinned + 1 kept
+  want = emcmc.step_kernel(4, q0, kernel=kern, seed=11)
+  assert torch.equal(q, want) and kr.call_counter == 1

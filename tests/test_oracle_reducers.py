@@ -156,3 +156,42 @@ def test_no_cpu_fallback():
     pytest.skip('GPU present')
   with pytest.raises(_lib.HipmcError):
     estats.RunningVariance.from_shape((3,))
+
+
+def test_sample_discarding_and_step_kernel_schedule():
+  """Burn-in and thinning (`sample_discarding_kernel.py:81-93`) and the
+  per-step seeds of step_kernel, on a counting kernel (no GPU)."""
+  from probability_amd import random as samplers
+  from probability_amd.experimental import mcmc as emcmc
+  from probability_amd.mcmc import kernel as kernel_base
+
+  class Count(kernel_base.TransitionKernel):
+    seeds = []
+
+    @property
+    def is_calibrated(self):
+      return True
+
+    def one_step(self, state, kr, seed=None):
+      Count.seeds.append(None if seed is None else tuple(int(v) for v in seed))
+      return state + 1, kr + 1
+
+    def bootstrap_results(self, state):
+      return 0
+
+  k = emcmc.SampleDiscardingKernel(Count(), num_burnin_steps=3,
+                                   num_steps_between_results=2)
+  kr = k.bootstrap_results(0)
+  assert kr.call_counter == 0
+  s, kr = k.one_step(0, kr)
+  assert (s, kr.call_counter, kr.inner_results) == (6, 1, 6)
+  s, kr = k.one_step(s, kr)
+  assert (s, kr.call_counter) == (9, 2)
+  Count.seeds = []
+  s, kr = emcmc.step_kernel(3, 0, kernel=Count(), seed=5,
+                            return_final_kernel_results=True)
+  base = samplers.sanitize_seed(5, salt='experimental.mcmc.step_kernel')
+  want = [tuple(int(v) for v in samplers.fold_in(base, t)) for t in range(3)]
+  assert (s, Count.seeds) == (3, want)
+  with pytest.raises(ValueError):
+    emcmc.SampleDiscardingKernel(Count(), num_burnin_steps=-1)
