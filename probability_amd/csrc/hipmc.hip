@@ -1,7 +1,9 @@
-// C ABI of libhipmc: argument checks, launch configuration and the small
-// auxiliary kernels (dual averaging, MH accept, bijectors, raw draws).
-// No C++ exception crosses this boundary; errors are negative return codes
-// plus a thread-local message.
+// C ABI of libhipmc: argument checks, dispatch to the sampling-kernel
+// launchers (built per dtype and target family in inst.hip; the BF16 MFMA
+// path in mfma.hip) and the small auxiliary kernels (dual averaging, MH
+// accept, bijectors, raw draws, running moments).  No C++ exception crosses
+// this boundary; errors are negative return codes plus a thread-local
+// message.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -11,8 +13,7 @@
 
 #include "../../include/hipmc.h"
 #include "common.h"
-#include "glm_mfma.h"
-#include "kernels.h"
+#include "runtime.h"
 #include "targets.h"
 
 using namespace hipmc;
@@ -23,9 +24,11 @@ static_assert(sizeof(hipmc_stream) == 48, "hipmc_stream layout");
 static_assert(offsetof(hipmc_problem, param) == 56, "hipmc_problem layout");
 static_assert(offsetof(hipmc_problem, n_bij) == 96, "hipmc_problem layout");
 
-namespace {
+namespace hipmc {
 
+namespace {
 thread_local std::string g_err;
+}  // namespace
 
 int fail(int code, const std::string& msg) {
   g_err = msg;
@@ -37,25 +40,6 @@ int hip_check(hipError_t e, const char* what) {
   return fail(HIPMC_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-hipStream_t as_stream(uint64_t s) { return reinterpret_cast<hipStream_t>(s); }
-
-constexpr int kMaxLdsPerBlock = 64 * 1024;
-// gfx950 gives one workgroup up to 160 KiB of LDS; a launch that asks for
-// more than 64 KiB raises the kernel's dynamic-LDS limit first.
-constexpr int kMaxLdsSharedChain = 160 * 1024;
-
-template <typename F>
-int allow_lds(F* kernel, size_t bytes) {
-  if (bytes > (size_t)kMaxLdsSharedChain)
-    return fail(HIPMC_E_UNSUPPORTED, "shared-chain LDS does not fit");
-  if (bytes <= (size_t)kMaxLdsPerBlock) return HIPMC_OK;
-  return hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       (int)bytes),
-                   "hipFuncSetAttribute(max dynamic LDS)");
-}
-
-// Smallest compiled slot count K with K*64 >= max_dim.
 int pick_k(int max_dim) {
   static const int ks[] = {1, 2, 3, 4, 5, 8};
   for (int k : ks)
@@ -63,11 +47,9 @@ int pick_k(int max_dim) {
   return -1;
 }
 
-int waves_for_lds(size_t per_wave_bytes) {
-  int w = 4;
-  while (w > 1 && (size_t)w * per_wave_bytes > (size_t)kMaxLdsPerBlock) --w;
-  return w;
-}
+}  // namespace hipmc
+
+namespace {
 
 int check_run(const hipmc_run_args* a, bool nuts) {
   if (!a) return fail(HIPMC_E_INVALID, "args is NULL");
@@ -99,130 +81,44 @@ int check_run(const hipmc_run_args* a, bool nuts) {
   return HIPMC_OK;
 }
 
-template <typename T, int K, int FAM>
-int launch_nuts(const hipmc_run_args& a0, hipStream_t s) {
-  if (a0.n_chains == 0) return HIPMC_OK;
-  // one chain per block: one U-turn memory, W target scratches and the two
-  // halves of the partial-sum buffer; fewer waves share the chain when that
-  // does not fit
-  auto shared_bytes = [&](int w) {
-    return ((size_t)2 * a0.max_tree_depth * K * 64 +
-            (size_t)w * target_scratch_elems<T, K, FAM>()) * sizeof(T) +
-           2 * glm_red_bytes<T, K>(w);
-  };
-  hipmc_run_args a = a0;
-  int W = waves_per_chain<FAM>(a);
-  while (W > 1 && shared_bytes(W) > (size_t)kMaxLdsSharedChain) --W;
-  a.waves_per_chain = W;
-  if (W > 1) {
-    const size_t bytes = shared_bytes(W);
-    const int rc = allow_lds(&nuts_kernel<T, K, FAM>, bytes);
-    if (rc != HIPMC_OK) return rc;
-    hipLaunchKernelGGL((nuts_kernel<T, K, FAM>), dim3(a.n_chains),
-                       dim3(64 * W), bytes, s, a);
-    return hip_check(hipGetLastError(), "nuts_kernel launch");
-  }
-  const size_t per_wave = nuts_lds_elems<T, K, FAM>(a.max_tree_depth) * sizeof(T);
-  if (per_wave > (size_t)kMaxLdsPerBlock)
-    return fail(HIPMC_E_UNSUPPORTED, "U-turn memory does not fit in LDS");
-  const int w = waves_for_lds(per_wave);
-  const int blocks = (a.n_chains + w - 1) / w;
-  hipLaunchKernelGGL((nuts_kernel<T, K, FAM>), dim3(blocks), dim3(64 * w),
-                     per_wave * w, s, a);
-  return hip_check(hipGetLastError(), "nuts_kernel launch");
-}
 
-template <typename T, int K, int FAM, bool CAL>
-int launch_hmc(const hipmc_run_args& a0, hipStream_t s) {
-  if (a0.n_chains == 0) return HIPMC_OK;
-  const size_t per_wave = hmc_lds_elems<T, K, FAM>() * sizeof(T);
-  hipmc_run_args a = a0;
-  // hmc_kernel is built for blocks of at most 256 threads (it may use more
-  // than 256 registers per lane, one wave per SIMD), so a shared chain
-  // spans at most four waves
-  int W = waves_per_chain<FAM>(a);
-  if (W > 4) W = 4;
-  while (W > 1 && per_wave * W + 2 * glm_red_bytes<T, K>(W) >
-                      (size_t)kMaxLdsSharedChain)
-    --W;
-  a.waves_per_chain = W;
-  if (W > 1) {
-    const size_t bytes = per_wave * W + 2 * glm_red_bytes<T, K>(W);
-    const int rc = allow_lds(&hmc_kernel<T, K, FAM, CAL>, bytes);
-    if (rc != HIPMC_OK) return rc;
-    hipLaunchKernelGGL((hmc_kernel<T, K, FAM, CAL>), dim3(a.n_chains),
-                       dim3(64 * W), bytes, s, a);
-    return hip_check(hipGetLastError(), "hmc_kernel launch");
-  }
-  const int w = 4;
-  const int blocks = (a.n_chains + w - 1) / w;
-  hipLaunchKernelGGL((hmc_kernel<T, K, FAM, CAL>), dim3(blocks), dim3(64 * w),
-                     per_wave * w, s, a);
-  return hip_check(hipGetLastError(), "hmc_kernel launch");
-}
-
-template <typename T, int FAM, template <typename, int, int> class F,
-          typename... A>
-int dispatch_k(int k, A&&... args) {
-  switch (k) {
-    case 1: return F<T, 1, FAM>::run(args...);
-    case 2: return F<T, 2, FAM>::run(args...);
-    case 3: return F<T, 3, FAM>::run(args...);
-    case 4: return F<T, 4, FAM>::run(args...);
-    case 5: return F<T, 5, FAM>::run(args...);
-    case 8: return F<T, 8, FAM>::run(args...);
-  }
-  return fail(HIPMC_E_UNSUPPORTED, "no kernel for this dimension");
-}
-
-template <template <typename, int, int> class F, typename... A>
-int dispatch(int dtype, int kind, int k, A&&... args) {
+// Calls L<T, FAM>::run(args...) for the launch's dtype and target family.
+template <template <typename, int> class L, typename... A>
+int by_family(int dtype, int kind, A&&... args) {
   if (kind < HIPMC_TARGET_LOGISTIC || kind > HIPMC_TARGET_NAN_EXP)
     return fail(HIPMC_E_INVALID, "unknown target kind");
   const int fam = family_of(kind);
   if (dtype == HIPMC_F32) {
-    if (fam == kFamilyHier) return dispatch_k<float, kFamilyHier, F>(k, args...);
-    if (fam == kFamilyGlm) return dispatch_k<float, kFamilyGlm, F>(k, args...);
-    return dispatch_k<float, kFamilyTest, F>(k, args...);
+    if (fam == kFamilyHier) return L<float, kFamilyHier>::run(args...);
+    if (fam == kFamilyGlm) return L<float, kFamilyGlm>::run(args...);
+    return L<float, kFamilyTest>::run(args...);
   }
   if (dtype == HIPMC_F64) {
-    if (fam == kFamilyHier) return dispatch_k<double, kFamilyHier, F>(k, args...);
-    if (fam == kFamilyGlm) return dispatch_k<double, kFamilyGlm, F>(k, args...);
-    return dispatch_k<double, kFamilyTest, F>(k, args...);
+    if (fam == kFamilyHier) return L<double, kFamilyHier>::run(args...);
+    if (fam == kFamilyGlm) return L<double, kFamilyGlm>::run(args...);
+    return L<double, kFamilyTest>::run(args...);
   }
   return fail(HIPMC_E_INVALID, "bad dtype");
 }
 
-template <typename T, int K, int FAM>
+template <typename T, int FAM>
 struct NutsL {
   static int run(const hipmc_run_args& a, hipStream_t s) {
-    return launch_nuts<T, K, FAM>(a, s);
+    return launch_nuts_k<T, FAM>(pick_k(a.max_dim), a, s);
   }
 };
-template <typename T, int K, int FAM>
+template <typename T, int FAM>
 struct HmcL {
-  static int run(const hipmc_run_args& a, hipStream_t s) {
-    return launch_hmc<T, K, FAM, true>(a, s);
+  static int run(bool calibrated, const hipmc_run_args& a, hipStream_t s) {
+    return launch_hmc_k<T, FAM>(pick_k(a.max_dim), calibrated, a, s);
   }
 };
-template <typename T, int K, int FAM>
-struct LeapL {
-  static int run(const hipmc_run_args& a, hipStream_t s) {
-    return launch_hmc<T, K, FAM, false>(a, s);
-  }
-};
-template <typename T, int K, int FAM>
+template <typename T, int FAM>
 struct EvalL {
-  static int run(const hipmc_problem* p, const int32_t* cp, int n, int stride,
-                 const void* x, void* tlp, void* grad, hipStream_t s) {
-    const size_t per_wave = hmc_lds_elems<T, K, FAM>() * sizeof(T);
-    const int w = 4;
-    const int blocks = (n + w - 1) / w;
-    if (blocks == 0) return HIPMC_OK;
-    hipLaunchKernelGGL((eval_kernel<T, K, FAM>), dim3(blocks), dim3(64 * w),
-                       per_wave * w, s, p, cp, n, stride, (const T*)x,
-                       (T*)tlp, (T*)grad);
-    return hip_check(hipGetLastError(), "eval_kernel launch");
+  static int run(int k, const hipmc_problem* p, const int32_t* cp, int n,
+                 int stride, const void* x, void* tlp, void* grad,
+                 hipStream_t s) {
+    return launch_eval_k<T, FAM>(k, p, cp, n, stride, x, tlp, grad, s);
   }
 };
 
@@ -628,80 +524,27 @@ int hipmc_sizeof_glm_hmc_args(void) {
 }
 
 int hipmc_glm_hmc_run(const hipmc_glm_hmc_args* a, uint64_t stream) {
-  if (!a) return fail(HIPMC_E_INVALID, "glm_hmc: args is NULL");
-  if (a->dim != kMD)
-    return fail(HIPMC_E_UNSUPPORTED, "glm_hmc: dim must be 256");
-  if (a->n_obs <= 0 || a->n_obs % kMRows)
-    return fail(HIPMC_E_INVALID, "glm_hmc: n_obs must be a positive multiple of 32");
-  if (a->n_chains <= 0 || a->n_chains % kMChains)
-    return fail(HIPMC_E_INVALID, "glm_hmc: n_chains must be a positive multiple of 64");
-  if (a->n_split < 1) return fail(HIPMC_E_INVALID, "glm_hmc: n_split must be >= 1");
-  if (!a->x_bf16 || !a->xt_bf16 || !a->y || !a->step_size || !a->state ||
-      !a->tlp || !a->grad || !a->ws_g || !a->ws_ll || !a->ws_q || !a->ws_p ||
-      !a->ws_b || !a->ws_ke0)
-    return fail(HIPMC_E_INVALID, "glm_hmc: NULL argument");
-  if (!a->want_grad_only) {
-    if (a->num_leapfrog_steps < 1)
-      return fail(HIPMC_E_INVALID, "glm_hmc: num_leapfrog_steps must be >= 1");
-    if (a->n_iters < 1) return fail(HIPMC_E_INVALID, "glm_hmc: n_iters must be >= 1");
-    if (!a->fold_seed && a->n_iters != 1)
-      return fail(HIPMC_E_INVALID, "glm_hmc: n_iters > 1 requires fold_seed");
-  }
-  hipStream_t s = as_stream(stream);
-  // rows per split: a multiple of the tile height
-  int rps = (a->n_obs + a->n_split - 1) / a->n_split;
-  rps = (rps + kMRows - 1) / kMRows * kMRows;
-  const int n_split = (a->n_obs + rps - 1) / rps;
-  if (n_split != a->n_split)
-    return fail(HIPMC_E_INVALID, "glm_hmc: n_split leaves an empty split; use fewer");
-  const dim3 ggrid(a->n_chains / kMChains, n_split);
-  const dim3 cgrid((a->n_chains + 3) / 4);
-  auto grad = [&](bool ll) {
-    hipLaunchKernelGGL(glm_grad_mfma, ggrid, dim3(256), 0, s,
-                       (const __bf16*)a->x_bf16, (const __bf16*)a->xt_bf16,
-                       a->y, a->n_obs, (const __bf16*)a->ws_b,
-                       (const __bf16*)a->ws_b + (size_t)a->n_chains * kMD,
-                       a->n_chains, rps, a->ws_g, ll ? a->ws_ll : nullptr);
-  };
-  if (a->want_grad_only) {
-    const size_t ne = (size_t)a->n_chains * kMD;
-    hipLaunchKernelGGL(mfma_prepare, dim3(blocks_for(ne, 256)), dim3(256), 0,
-                       s, *a);
-    grad(true);
-    hipLaunchKernelGGL(mfma_chain_step, cgrid, dim3(256), 0, s, *a, 3, 0);
-    return hip_check(hipGetLastError(), "glm_hmc bootstrap launch");
-  }
-  for (int it = 0; it < a->n_iters; ++it) {
-    hipLaunchKernelGGL(mfma_chain_step, cgrid, dim3(256), 0, s, *a, 0, it);
-    for (int st = 0; st < a->num_leapfrog_steps; ++st) {
-      const bool last = st + 1 == a->num_leapfrog_steps;
-      grad(last);
-      hipLaunchKernelGGL(mfma_chain_step, cgrid, dim3(256), 0, s, *a,
-                         last ? 2 : 1, it);
-    }
-  }
-  return hip_check(hipGetLastError(), "glm_hmc launch");
+  return launch_glm_hmc(a, as_stream(stream));
 }
 
 int hipmc_nuts_run(const hipmc_run_args* a, uint64_t stream) {
   int rc = check_run(a, true);
   if (rc) return rc;
-  return dispatch<NutsL>(a->dtype, a->target_kind, pick_k(a->max_dim), *a,
-                         as_stream(stream));
+  return by_family<NutsL>(a->dtype, a->target_kind, *a, as_stream(stream));
 }
 
 int hipmc_hmc_run(const hipmc_run_args* a, uint64_t stream) {
   int rc = check_run(a, false);
   if (rc) return rc;
-  return dispatch<HmcL>(a->dtype, a->target_kind, pick_k(a->max_dim), *a,
-                        as_stream(stream));
+  return by_family<HmcL>(a->dtype, a->target_kind, true, *a,
+                         as_stream(stream));
 }
 
 int hipmc_leapfrog_run(const hipmc_run_args* a, uint64_t stream) {
   int rc = check_run(a, false);
   if (rc) return rc;
   if (a->n_iters != 1) return fail(HIPMC_E_INVALID, "leapfrog_run: n_iters must be 1");
-  return dispatch<LeapL>(a->dtype, a->target_kind, pick_k(a->max_dim), *a,
+  return by_family<HmcL>(a->dtype, a->target_kind, false, *a,
                          as_stream(stream));
 }
 
@@ -715,7 +558,7 @@ int hipmc_target_eval(int32_t dtype, int32_t target_kind,
     return fail(HIPMC_E_INVALID, "need 1 <= max_dim <= dim_stride");
   const int k = pick_k(max_dim);
   if (k < 0) return fail(HIPMC_E_UNSUPPORTED, "state dimension above 512 is not built");
-  return dispatch<EvalL>(dtype, target_kind, k, problems, chain_problem,
+  return by_family<EvalL>(dtype, target_kind, k, problems, chain_problem,
                          n_chains, dim_stride, x, out_tlp, out_grad,
                          as_stream(stream));
 }

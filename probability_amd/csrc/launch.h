@@ -1,0 +1,146 @@
+// Launch configuration of the sampling kernels (nuts_kernel, hmc_kernel,
+// eval_kernel) for one (dtype, family) pair; included by inst.hip only.
+#pragma once
+
+#include "common.h"
+#include "kernels.h"
+#include "runtime.h"
+#include "targets.h"
+
+namespace hipmc {
+
+template <typename F>
+int allow_lds(F* kernel, size_t bytes) {
+  if (bytes > (size_t)kMaxLdsSharedChain)
+    return fail(HIPMC_E_UNSUPPORTED, "shared-chain LDS does not fit");
+  if (bytes <= (size_t)kMaxLdsPerBlock) return HIPMC_OK;
+  return hip_check(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)bytes),
+                   "hipFuncSetAttribute(max dynamic LDS)");
+}
+
+inline int waves_for_lds(size_t per_wave_bytes) {
+  int w = 4;
+  while (w > 1 && (size_t)w * per_wave_bytes > (size_t)kMaxLdsPerBlock) --w;
+  return w;
+}
+
+template <typename T, int K, int FAM>
+int launch_nuts(const hipmc_run_args& a0, hipStream_t s) {
+  if (a0.n_chains == 0) return HIPMC_OK;
+  // one chain per block: one U-turn memory, W target scratches and the two
+  // halves of the partial-sum buffer; fewer waves share the chain when that
+  // does not fit
+  auto shared_bytes = [&](int w) {
+    return ((size_t)2 * a0.max_tree_depth * K * 64 +
+            (size_t)w * target_scratch_elems<T, K, FAM>()) * sizeof(T) +
+           2 * glm_red_bytes<T, K>(w);
+  };
+  hipmc_run_args a = a0;
+  int W = waves_per_chain<FAM>(a);
+  while (W > 1 && shared_bytes(W) > (size_t)kMaxLdsSharedChain) --W;
+  a.waves_per_chain = W;
+  if (W > 1) {
+    const size_t bytes = shared_bytes(W);
+    const int rc = allow_lds(&nuts_kernel<T, K, FAM>, bytes);
+    if (rc != HIPMC_OK) return rc;
+    hipLaunchKernelGGL((nuts_kernel<T, K, FAM>), dim3(a.n_chains),
+                       dim3(64 * W), bytes, s, a);
+    return hip_check(hipGetLastError(), "nuts_kernel launch");
+  }
+  const size_t per_wave = nuts_lds_elems<T, K, FAM>(a.max_tree_depth) * sizeof(T);
+  if (per_wave > (size_t)kMaxLdsPerBlock)
+    return fail(HIPMC_E_UNSUPPORTED, "U-turn memory does not fit in LDS");
+  const int w = waves_for_lds(per_wave);
+  const int blocks = (a.n_chains + w - 1) / w;
+  hipLaunchKernelGGL((nuts_kernel<T, K, FAM>), dim3(blocks), dim3(64 * w),
+                     per_wave * w, s, a);
+  return hip_check(hipGetLastError(), "nuts_kernel launch");
+}
+
+template <typename T, int K, int FAM, bool CAL>
+int launch_hmc(const hipmc_run_args& a0, hipStream_t s) {
+  if (a0.n_chains == 0) return HIPMC_OK;
+  const size_t per_wave = hmc_lds_elems<T, K, FAM>() * sizeof(T);
+  hipmc_run_args a = a0;
+  // hmc_kernel is built for blocks of at most 256 threads (it may use more
+  // than 256 registers per lane, one wave per SIMD), so a shared chain
+  // spans at most four waves
+  int W = waves_per_chain<FAM>(a);
+  if (W > 4) W = 4;
+  while (W > 1 && per_wave * W + 2 * glm_red_bytes<T, K>(W) >
+                      (size_t)kMaxLdsSharedChain)
+    --W;
+  a.waves_per_chain = W;
+  if (W > 1) {
+    const size_t bytes = per_wave * W + 2 * glm_red_bytes<T, K>(W);
+    const int rc = allow_lds(&hmc_kernel<T, K, FAM, CAL>, bytes);
+    if (rc != HIPMC_OK) return rc;
+    hipLaunchKernelGGL((hmc_kernel<T, K, FAM, CAL>), dim3(a.n_chains),
+                       dim3(64 * W), bytes, s, a);
+    return hip_check(hipGetLastError(), "hmc_kernel launch");
+  }
+  const int w = 4;
+  const int blocks = (a.n_chains + w - 1) / w;
+  hipLaunchKernelGGL((hmc_kernel<T, K, FAM, CAL>), dim3(blocks), dim3(64 * w),
+                     per_wave * w, s, a);
+  return hip_check(hipGetLastError(), "hmc_kernel launch");
+}
+
+template <typename T, int K, int FAM>
+int launch_eval(const hipmc_problem* p, const int32_t* cp, int n, int stride,
+                const void* x, void* tlp, void* grad, hipStream_t s) {
+  const size_t per_wave = hmc_lds_elems<T, K, FAM>() * sizeof(T);
+  const int w = 4;
+  const int blocks = (n + w - 1) / w;
+  if (blocks == 0) return HIPMC_OK;
+  hipLaunchKernelGGL((eval_kernel<T, K, FAM>), dim3(blocks), dim3(64 * w),
+                     per_wave * w, s, p, cp, n, stride, (const T*)x,
+                     (T*)tlp, (T*)grad);
+  return hip_check(hipGetLastError(), "eval_kernel launch");
+}
+
+// The slot counts built for every (dtype, family) pair; pick_k (hipmc.hip)
+// chooses the smallest that holds the state.
+#define HIPMC_FOR_EACH_K(X) X(1) X(2) X(3) X(4) X(5) X(8)
+
+template <typename T, int FAM>
+int launch_nuts_k(int k, const hipmc_run_args& a, hipStream_t s) {
+  switch (k) {
+#define HIPMC_CASE(KK) \
+  case KK: return launch_nuts<T, KK, FAM>(a, s);
+    HIPMC_FOR_EACH_K(HIPMC_CASE)
+#undef HIPMC_CASE
+  }
+  return fail(HIPMC_E_UNSUPPORTED, "no kernel for this dimension");
+}
+
+template <typename T, int FAM>
+int launch_hmc_k(int k, bool calibrated, const hipmc_run_args& a,
+                 hipStream_t s) {
+  switch (k) {
+#define HIPMC_CASE(KK)                                        \
+  case KK:                                                    \
+    return calibrated ? launch_hmc<T, KK, FAM, true>(a, s)    \
+                      : launch_hmc<T, KK, FAM, false>(a, s);
+    HIPMC_FOR_EACH_K(HIPMC_CASE)
+#undef HIPMC_CASE
+  }
+  return fail(HIPMC_E_UNSUPPORTED, "no kernel for this dimension");
+}
+
+template <typename T, int FAM>
+int launch_eval_k(int k, const hipmc_problem* p, const int32_t* cp, int n,
+                  int stride, const void* x, void* tlp, void* grad,
+                  hipStream_t s) {
+  switch (k) {
+#define HIPMC_CASE(KK) \
+  case KK: return launch_eval<T, KK, FAM>(p, cp, n, stride, x, tlp, grad, s);
+    HIPMC_FOR_EACH_K(HIPMC_CASE)
+#undef HIPMC_CASE
+  }
+  return fail(HIPMC_E_UNSUPPORTED, "no kernel for this dimension");
+}
+
+}  // namespace hipmc
