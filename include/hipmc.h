@@ -56,7 +56,7 @@
 extern "C" {
 #endif
 
-#define HIPMC_ABI_VERSION 2
+#define HIPMC_ABI_VERSION 3
 
 /* error codes */
 #define HIPMC_OK 0
@@ -224,6 +224,21 @@ typedef struct hipmc_run_args {
   /* optional row of chain c in state / tlp / grad (device, [n_chains]);
    * NULL: row c.  Lets a launch cover a subset of a resident chain pool. */
   const int32_t* chain_row;
+  /* optional per-iteration target log-prob after the step (dtype,
+   * [rows, n_chains], written like out_log_accept_ratio) */
+  void* out_target_log_prob;
+  /* optional diagonal momentum precision (device, dtype,
+   * [n_problems, dim_stride], row chain_problem[c]): the precision of the
+   * momentum distribution N(0, diag(1 / prec)), i.e. the estimated
+   * posterior variance of each state element, as set by
+   * DiagonalMassMatrixAdaptation
+   * (experimental/mcmc/diagonal_mass_matrix_adaptation.py:227-240,
+   * preconditioning_utils.py:70-88).  Momentum p = z / sqrt(prec),
+   * velocity prec * p, kinetic energy 0.5 sum(prec p^2) (the preconditioned
+   * kernels, experimental/mcmc/preconditioned_hmc.py,
+   * preconditioned_nuts.py:804-1010); NUTS then reports energy including
+   * the momentum density's normalising terms.  NULL: identity. */
+  const void* momentum_precision;
 } hipmc_run_args;
 
 /*

@@ -80,6 +80,7 @@ class RunArgs(ctypes.Structure):
       ('out_proposed_state', _vp), ('out_proposed_tlp', _vp),
       ('out_initial_momentum', _vp), ('out_final_momentum', _vp),
       ('streams', _vp), ('chain_row', _vp),
+      ('out_target_log_prob', _vp), ('momentum_precision', _vp),
   ]
 
 

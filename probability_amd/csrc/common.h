@@ -256,6 +256,17 @@ __device__ inline void wave_sum4(A a, A b, A c, A d, A& ra, A& rb, A& rc,
 }
 
 // ---------------------------------------------------------------- scalars
+// Read-only pointer in the global address space.  Loads through it compile to
+// global loads, which count only against vmcnt; loads through a generic
+// pointer (one read from a table in memory) compile to flat loads, which
+// also count against lgkmcnt, so every LDS wait would wait for them too.
+template <typename T>
+using gptr = const __attribute__((address_space(1))) T*;
+// Pointer in the LDS address space (ds_read / ds_write; through a generic
+// pointer the compiler emits flat accesses).
+template <typename T>
+using lptr = __attribute__((address_space(3))) T*;
+
 __device__ inline double neg_inf() { return -__builtin_inf(); }
 
 // An empty volatile asm: first in a wave-uniform branch whose body is

@@ -181,24 +181,97 @@ __device__ inline void store_row(T* row, const T (&v)[K], int dim) {
 }
 
 // `num_steps` leapfrog steps with the reference op order
-// (leapfrog_integrator.py:280-309, 319-362):
-//   p += (0.5*eps)*g; repeat: q += eps*p; (t, g) = grad(q); p += eps*g;
-//   p -= (0.5*eps)*g.
+// (leapfrog_integrator.py:280-309, 319-362), positions moving with the
+// velocity prec * p of the momentum distribution (identity: prec = 1):
+//   p += (0.5*eps)*g; repeat: q += eps*(prec*p); (t, g) = grad(q);
+//   p += eps*g; p -= (0.5*eps)*g.
 template <typename T, int K, int FAM>
 __device__ inline void leapfrog(Target<T, K, FAM>& tg, T eps, int num_steps,
-                                T (&q)[K], T (&p)[K], T (&g)[K], double& t) {
+                                const T (&m)[K], T (&q)[K], T (&p)[K],
+                                T (&g)[K], double& t) {
   const T half = T(0.5) * eps;
 #pragma unroll
   for (int k = 0; k < K; ++k) p[k] = p[k] + half * g[k];
   for (int s = 0; s < num_steps; ++s) {
 #pragma unroll
-    for (int k = 0; k < K; ++k) q[k] = q[k] + eps * p[k];
+    for (int k = 0; k < K; ++k) q[k] = q[k] + eps * (m[k] * p[k]);
     t = tg.eval(q, g);
 #pragma unroll
     for (int k = 0; k < K; ++k) p[k] = p[k] + eps * g[k];
   }
 #pragma unroll
   for (int k = 0; k < K; ++k) p[k] = p[k] - half * g[k];
+}
+
+// Diagonal momentum precision of the chain (hipmc_run_args
+// .momentum_precision; identity when absent): prec[e] for the chain's
+// elements (1 past dim), and the momentum density's normalising terms
+// -0.5 dim log(2 pi) + 0.5 sum log prec, which the preconditioned NUTS
+// includes in its energy (preconditioned_nuts.py:1091-1099, the log-density
+// of MultivariateNormalPrecisionFactorLinearOperator).
+template <typename T, int K>
+__device__ inline bool load_precision(const hipmc_run_args& a,
+                                      const ChainId& id, int dim, T (&m)[K],
+                                      double& kconst) {
+  const int lane = lane_id();
+  kconst = 0.0;
+  if (!a.momentum_precision) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) m[k] = T(1);
+    return false;
+  }
+  const T* row = (const T*)a.momentum_precision + (size_t)id.prob * a.dim_stride;
+  double lsum = 0.0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int e = k * 64 + lane;
+    m[k] = e < dim ? row[e] : T(1);
+    lsum += e < dim ? log((double)m[k]) : 0.0;
+  }
+  kconst = 0.5 * wave_sum(lsum) - 0.5 * (double)dim * log(2.0 * M_PI);
+  return true;
+}
+
+// Momentum of the preconditioned kernels: p = z / sqrt(prec)
+// (MultivariateNormalPrecisionFactorLinearOperator samples x = F^-T w with
+// F = diag(sqrt(prec)), mvn_precision_factor_linop.py:171-211).
+template <typename T, int K>
+__device__ inline void precondition_momentum(const T (&m)[K], T (&p)[K]) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) p[k] = p[k] / sqrt(m[k]);
+}
+
+// 0.5 * sum(prec * p^2), float64 (the kinetic energy without the normalising
+// terms).
+template <typename T, int K>
+__device__ inline double half_kinetic(const T (&m)[K], const T (&p)[K]) {
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) s += (double)p[k] * (double)(m[k] * p[k]);
+  return 0.5 * wave_sum(s);
+}
+
+// velocity prec * p
+template <typename T, int K>
+__device__ inline void velocity(T (&v)[K], const T (&m)[K], const T (&p)[K]) {
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = m[k] * p[k];
+}
+
+// Lane-wise copies of a register array to and from an LDS slot of K*64
+// elements: each lane writes and reads back only its own elements, so no
+// wave barrier is needed between them.
+template <typename T, int K>
+__device__ inline void park(T* slot, const T (&v)[K]) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int k = 0; k < K; ++k) slot[k * 64 + lane] = v[k];
+}
+template <typename T, int K>
+__device__ inline void unpark(T (&v)[K], const T* slot) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = slot[k * 64 + lane];
 }
 
 template <typename T>
@@ -222,21 +295,58 @@ __host__ __device__ inline int waves_per_chain(const hipmc_run_args& a) {
 }
 
 // ------------------------------------------------------------------ NUTS
-// LDS per wave (elements of T): U-turn memory momentum [max_depth][K*64],
-// cumulative momentum [max_depth][K*64], then the target's scratch.
+// GLM chains hold their long-lived vectors in LDS ("parked"): the gradient
+// evaluation keeps many loads in flight and needs most of the 256 registers
+// that two waves per SIMD allow.  The trajectory ends, the candidate, the
+// transition's start and the momentum sum change once per doubling and sit
+// in LDS for the whole leaf loop (kTreeSlots slots, one copy per chain); the
+// leaf's momentum, momentum sum and subtree candidate are parked around
+// each gradient evaluation (kLeafSlots slots per wave).  The other families
+// keep everything in registers (one wave per SIMD).
+constexpr int kTreeSlots = 11;
+constexpr int kLeafSlots = 4;
+template <int FAM>
+constexpr bool parks() { return FAM == kFamilyGlm; }
+
+// LDS of one chain (elements of T): U-turn memory velocity [max_depth][K*64]
+// and cumulative momentum [max_depth][K*64], the parked tree vectors
+// [kTreeSlots][K*64] (GLM); then per wave the target's scratch and the
+// parked leaf vectors [kLeafSlots][K*64] (GLM).
+template <typename T, int K, int FAM>
+__host__ __device__ constexpr size_t nuts_chain_lds_elems(int max_depth) {
+  return (size_t)2 * max_depth * K * 64 +
+         (parks<FAM>() ? (size_t)kTreeSlots * K * 64 : 0);
+}
+template <typename T, int K, int FAM>
+__host__ __device__ constexpr size_t nuts_wave_lds_elems() {
+  return target_scratch_elems<T, K, FAM>() +
+         (parks<FAM>() ? (size_t)kLeafSlots * K * 64 : 0);
+}
+// LDS of one wave when every wave holds its own chain.
 template <typename T, int K, int FAM>
 __host__ __device__ constexpr size_t nuts_lds_elems(int max_depth) {
-  return (size_t)2 * max_depth * K * 64 + target_scratch_elems<T, K, FAM>();
+  return nuts_chain_lds_elems<T, K, FAM>(max_depth) +
+         nuts_wave_lds_elems<T, K, FAM>();
 }
 
 // One chain's transitions (the body of nuts_kernel).  BM: the target's
 // bijector code path; U1: one leapfrog step per tree leaf
 // (unrolled_leapfrog_steps == 1).  Specialising both keeps the leaf free of
 // run-time branches and of the registers the other paths would hold.
+// With a momentum precision (the preconditioned NUTS,
+// preconditioned_nuts.py:330-1010) the U-turn memory holds velocities and
+// the U-turn checks take momentum sums against velocities; with none, the
+// velocity is the momentum (nuts.py:541-1010).
+// W > 1: the W waves of one block share the chain (GLMs); they run the same
+// control flow on identical values, and their parked tree vectors are one
+// copy guarded by a block barrier before each write.
 template <typename T, int K, int FAM, int BM, bool U1>
 __device__ __attribute__((always_inline)) inline void nuts_chain(
     const hipmc_run_args& a, Target<T, K, FAM>& tg, const ChainId& id, int c,
-    int wic, T* memP, T* memC, const hipmc_problem& pb) {
+    int wic, int W, T* memP, T* memC, T* tree, T* leafp,
+    const hipmc_problem& pb) {
+  constexpr bool PARK = parks<FAM>();
+  constexpr size_t S = (size_t)K * 64;
   const int lane = lane_id();
   const int max_depth = a.max_tree_depth;
   const int dim = pb.dim;
@@ -245,6 +355,9 @@ __device__ __attribute__((always_inline)) inline void nuts_chain(
   const double max_de = a.max_energy_diff;
   const int nunroll = a.unrolled_leapfrog_steps;
 
+  T m[K];
+  double kconst;
+  const bool pre = load_precision(a, id, dim, m, kconst);
   T q0[K], g0[K];
   load_row(q0, (const T*)a.state + row, dim);
   load_row(g0, (const T*)a.grad + row, dim);
@@ -256,7 +369,8 @@ __device__ __attribute__((always_inline)) inline void nuts_chain(
     const Seed sd = iter_seed(stream, it);
     T pL[K], qL[K], gL[K], pR[K], qR[K], gR[K];
     draw_momentum(sd, id, dim, pL);
-    const double h0 = t0 - half_sq_norm(pL);  // compute_hamiltonian, -H
+    if (pre) precondition_momentum(m, pL);
+    const double h0 = t0 - half_kinetic(m, pL);  // compute_hamiltonian, -H
     copy(qL, q0);
     copy(gL, g0);
     copy(pR, pL);
@@ -289,6 +403,13 @@ __device__ __attribute__((always_inline)) inline void nuts_chain(
       } else {
         copy(q, qL); copy(p, pL); copy(g, gL); t = tL;
       }
+      if (PARK) {
+        if (W > 1) __syncthreads();  // every wave has read the last copy
+        park(tree + 0 * S, pL); park(tree + 1 * S, qL); park(tree + 2 * S, gL);
+        park(tree + 3 * S, pR); park(tree + 4 * S, qR); park(tree + 5 * S, gR);
+        park(tree + 6 * S, qC); park(tree + 7 * S, gC); park(tree + 8 * S, q0);
+        park(tree + 9 * S, g0); park(tree + 10 * S, rho);
+      }
       // _build_sub_tree (nuts.py:713-791)
       T qS[K], gS[K];
       copy(qS, q);
@@ -316,39 +437,59 @@ __device__ __attribute__((always_inline)) inline void nuts_chain(
           lu_buf = (double)log1m(uniform_from<T>(lw.x, lw.y));
         }
         double energy;
+        T v[K];
         if (U1 || nunroll == 1) {
           const T half = T(0.5) * e;
 #pragma unroll
           for (int k = 0; k < K; ++k) {
             p[k] = p[k] + half * g[k];
-            q[k] = q[k] + e * p[k];
+            q[k] = q[k] + e * (m[k] * p[k]);
+          }
+          if (PARK) {
+            park(leafp + 0 * S, p); park(leafp + 1 * S, cs);
+            park(leafp + 2 * S, qS); park(leafp + 3 * S, gS);
           }
           double lpu;
           const double lpl = tg.template eval_partial<BM>(q, g, lpu);
+          if (PARK) {
+            unpark(p, leafp + 0 * S); unpark(cs, leafp + 1 * S);
+            unpark(qS, leafp + 2 * S); unpark(gS, leafp + 3 * S);
+          }
           double kin = 0.0;
 #pragma unroll
           for (int k = 0; k < K; ++k) {
             p[k] = p[k] + e * g[k];
             p[k] = p[k] - half * g[k];
-            kin += (double)p[k] * (double)p[k];
+            v[k] = m[k] * p[k];
+            kin += (double)p[k] * (double)v[k];
           }
           double slp, skin;
           wave_sum2(lpl, kin, slp, skin);
           t = slp + lpu;
           energy = t - 0.5 * skin;
         } else {
-          leapfrog(tg, e, nunroll, q, p, g, t);
-          energy = t - half_sq_norm(p);
+          if (PARK) {
+            park(leafp + 1 * S, cs); park(leafp + 2 * S, qS);
+            park(leafp + 3 * S, gS);
+          }
+          leapfrog(tg, e, nunroll, m, q, p, g, t);
+          if (PARK) {
+            unpark(cs, leafp + 1 * S); unpark(qS, leafp + 2 * S);
+            unpark(gS, leafp + 3 * S);
+          }
+          velocity(v, m, p);
+          energy = t - half_kinetic(m, p);
         }
         ++lf_sub;
         if ((i & 1) == 0) {
-          // store (p, cumsum before this step) in slot popcount(i)
+          // store (velocity, momentum sum before this step) in slot
+          // popcount(i)
           const int slot = __popc(i);
           T* mp = memP + (size_t)slot * K * 64;
           T* mc = memC + (size_t)slot * K * 64;
 #pragma unroll
           for (int k = 0; k < K; ++k) {
-            mp[k * 64 + lane] = p[k];
+            mp[k * 64 + lane] = v[k];
             mc[k * 64 + lane] = cs[k];
           }
         }
@@ -378,9 +519,9 @@ __device__ __attribute__((always_inline)) inline void nuts_chain(
               const T d0 = cs[k] - mc0[k * 64 + lane];
               const T d1 = cs[k] - mc1[k * 64 + lane];
               sl0 += d0 * mp0[k * 64 + lane];
-              sr0 += d0 * p[k];
+              sr0 += d0 * v[k];
               sl1 += d1 * mp1[k * 64 + lane];
-              sr1 += d1 * p[k];
+              sr1 += d1 * v[k];
             }
             T dl0, dr0, dl1, dr1;
             wave_sum4(sl0, sr0, sl1, sr1, dl0, dr0, dl1, dr1);
@@ -411,6 +552,14 @@ __device__ __attribute__((always_inline)) inline void nuts_chain(
         }
         if (!csub) break;
       }
+      if (PARK) {
+        unpark(pL, tree + 0 * S); unpark(qL, tree + 1 * S);
+        unpark(gL, tree + 2 * S); unpark(pR, tree + 3 * S);
+        unpark(qR, tree + 4 * S); unpark(gR, tree + 5 * S);
+        unpark(qC, tree + 6 * S); unpark(gC, tree + 7 * S);
+        unpark(q0, tree + 8 * S); unpark(g0, tree + 9 * S);
+        unpark(rho, tree + 10 * S);
+      }
       eds += eds_sub;
       lf += lf_sub;
       const double tree_w = csub ? wS : neg_inf();
@@ -432,8 +581,11 @@ __device__ __attribute__((always_inline)) inline void nuts_chain(
       }
 #pragma unroll
       for (int k = 0; k < K; ++k) rho[k] = rho[k] + cs[k];
-      const T dl = dot(rho, pL);
-      const T dr = dot(rho, pR);
+      T vL[K], vR[K];
+      velocity(vL, m, pL);
+      velocity(vR, m, pR);
+      const T dl = dot(rho, vL);
+      const T dr = dot(rho, vR);
       is_acc = is_acc || choose;
       cont = csub && (dl >= T(0)) && (dr >= T(0));
       notdiv = nd;
@@ -450,8 +602,9 @@ __device__ __attribute__((always_inline)) inline void nuts_chain(
       if (a.out_is_accepted) a.out_is_accepted[o] = is_acc;
       if (a.out_reach_max_depth) a.out_reach_max_depth[o] = cont;
       if (a.out_has_divergence) a.out_has_divergence[o] = !notdiv;
-      put<T>(a.out_energy, o, (T)eC);
+      put<T>(a.out_energy, o, (T)(eC + kconst));
       if (a.out_tree_depth) a.out_tree_depth[o] = depth;
+      put<T>(a.out_target_log_prob, o, (T)t0);
     }
     if (wic == 0) trace_store(stream, a, id, dim, it, q0);
   }
@@ -481,27 +634,34 @@ __global__ void __launch_bounds__(FAM == kFamilyGlm ? 512 : 256)
                       : xcd_block(blockIdx.x, gridDim.x) * waves + wave;
   const int wic = W > 1 ? wave : 0;
   if (c >= a.n_chains) return;  // whole wave (block, if shared) leaves
-  const size_t umem = (size_t)2 * max_depth * K * 64;
+  const size_t cl = nuts_chain_lds_elems<T, K, FAM>(max_depth);
+  const size_t wl = nuts_wave_lds_elems<T, K, FAM>();
   const size_t scr = target_scratch_elems<T, K, FAM>();
-  T* wl = W > 1 ? (T*)smem_raw
-                : (T*)smem_raw + (size_t)wave * nuts_lds_elems<T, K, FAM>(max_depth);
-  T* memP = wl;  // one copy per chain; the waves of a chain write equal values
-  T* memC = wl + (size_t)max_depth * K * 64;
-  T* scratch = wl + umem + (size_t)wic * scr;
+  // W > 1: [chain block][W wave blocks][partial-sum halves];
+  // W == 1: one [chain block, wave block] per wave
+  T* chain_base = W > 1 ? (T*)smem_raw : (T*)smem_raw + (size_t)wave * (cl + wl);
+  T* wave_base = W > 1 ? (T*)smem_raw + cl + (size_t)wic * wl : chain_base + cl;
+  T* memP = chain_base;  // one copy per chain; the waves of a chain write equal values
+  T* memC = chain_base + (size_t)max_depth * K * 64;
+  T* tree = chain_base + (size_t)2 * max_depth * K * 64;
+  T* scratch = wave_base;
+  T* leafp = wave_base + scr;
 
   const ChainId id = chain_of(a, c);
   const hipmc_problem& pb = a.problems[id.prob];
   Target<T, K, FAM> tg;
   tg.load(pb, scratch);
-  if (W > 1) tg.set_split(wic, W, (unsigned char*)(wl + umem + (size_t)W * scr));
+  if (W > 1) tg.set_split(wic, W, (unsigned char*)((T*)smem_raw + cl + (size_t)W * wl));
   const bool u1 = a.unrolled_leapfrog_steps == 1;
   if (FAM == kFamilyHier && u1 && tg.sp0) {
-    nuts_chain<T, K, FAM, kBijSlot0Softplus, true>(a, tg, id, c, wic, memP,
-                                                    memC, pb);
+    nuts_chain<T, K, FAM, kBijSlot0Softplus, true>(a, tg, id, c, wic, W, memP,
+                                                    memC, tree, leafp, pb);
   } else if (u1) {
-    nuts_chain<T, K, FAM, kBijAny, true>(a, tg, id, c, wic, memP, memC, pb);
+    nuts_chain<T, K, FAM, kBijAny, true>(a, tg, id, c, wic, W, memP, memC,
+                                         tree, leafp, pb);
   } else {
-    nuts_chain<T, K, FAM, kBijAny, false>(a, tg, id, c, wic, memP, memC, pb);
+    nuts_chain<T, K, FAM, kBijAny, false>(a, tg, id, c, wic, W, memP, memC,
+                                          tree, leafp, pb);
   }
 }
 
@@ -537,6 +697,9 @@ __global__ void __launch_bounds__(256) hmc_kernel(hipmc_run_args a) {
   const size_t row = (size_t)crow * a.dim_stride;
   const int L = a.num_leapfrog_steps;
 
+  T m[K];
+  double kconst;
+  const bool pre = load_precision(a, id, dim, m, kconst);
   T q0[K], g0[K];
   load_row(q0, (const T*)a.state + row, dim);
   load_row(g0, (const T*)a.grad + row, dim);
@@ -552,14 +715,15 @@ __global__ void __launch_bounds__(256) hmc_kernel(hipmc_run_args a) {
     const Seed prop = CALIBRATED ? split(sd, 0) : sd;
     T p[K], q[K], g[K];
     draw_momentum(prop, id, dim, p);
+    if (pre) precondition_momentum(m, p);
     T p_init[K];
     copy(p_init, p);
-    const double ke0 = half_sq_norm(p);
+    const double ke0 = half_kinetic(m, p);
     copy(q, q0);
     copy(g, g0);
     double t = t0;
-    leapfrog(tg, eps, L, q, p, g, t);
-    const double ke1 = half_sq_norm(p);
+    leapfrog(tg, eps, L, m, q, p, g, t);
+    const double ke1 = half_kinetic(m, p);
     // _compute_log_acceptance_correction: 0.5*safe_sum([sum p0^2, -sum p1^2])
     double corr = ke0 - ke1;
     if (!__builtin_isfinite(corr)) corr = neg_inf();
@@ -588,6 +752,7 @@ __global__ void __launch_bounds__(256) hmc_kernel(hipmc_run_args a) {
         put<T>(a.out_log_accept_ratio, o, (T)lar);
         if (a.out_is_accepted) a.out_is_accepted[o] = accept;
         put<T>(a.out_energy, o, (T)corr);
+        put<T>(a.out_target_log_prob, o, (T)t0);
       }
     } else {
       copy(q0, q);
@@ -595,7 +760,10 @@ __global__ void __launch_bounds__(256) hmc_kernel(hipmc_run_args a) {
       t0 = (double)(T)t;
       if (writer && a.out_initial_momentum) store_row((T*)a.out_initial_momentum + row, p_init, dim);
       if (writer && a.out_final_momentum) store_row((T*)a.out_final_momentum + row, p, dim);
-      if (lane == 0 && writer) put<T>(a.out_energy, o, (T)corr);
+      if (lane == 0 && writer) {
+        put<T>(a.out_energy, o, (T)corr);
+        put<T>(a.out_target_log_prob, o, (T)t0);
+      }
     }
     if (writer) trace_store(stream, a, id, dim, it, q0);
   }

@@ -29,13 +29,14 @@ inline int waves_for_lds(size_t per_wave_bytes) {
 template <typename T, int K, int FAM>
 int launch_nuts(const hipmc_run_args& a0, hipStream_t s) {
   if (a0.n_chains == 0) return HIPMC_OK;
-  // one chain per block: one U-turn memory, W target scratches and the two
-  // halves of the partial-sum buffer; fewer waves share the chain when that
-  // does not fit
+  // one chain per block: the chain's LDS (U-turn memory, parked tree
+  // vectors), W wave blocks (target scratch, parked leaf vectors) and the
+  // two halves of the partial-sum buffer; fewer waves share the chain when
+  // that does not fit
+  const size_t cl = nuts_chain_lds_elems<T, K, FAM>(a0.max_tree_depth);
+  const size_t wl = nuts_wave_lds_elems<T, K, FAM>();
   auto shared_bytes = [&](int w) {
-    return ((size_t)2 * a0.max_tree_depth * K * 64 +
-            (size_t)w * target_scratch_elems<T, K, FAM>()) * sizeof(T) +
-           2 * glm_red_bytes<T, K>(w);
+    return (cl + (size_t)w * wl) * sizeof(T) + 2 * glm_red_bytes<T, K>(w);
   };
   hipmc_run_args a = a0;
   int W = waves_per_chain<FAM>(a);
@@ -49,7 +50,7 @@ int launch_nuts(const hipmc_run_args& a0, hipStream_t s) {
                        dim3(64 * W), bytes, s, a);
     return hip_check(hipGetLastError(), "nuts_kernel launch");
   }
-  const size_t per_wave = nuts_lds_elems<T, K, FAM>(a.max_tree_depth) * sizeof(T);
+  const size_t per_wave = (cl + wl) * sizeof(T);
   if (per_wave > (size_t)kMaxLdsPerBlock)
     return fail(HIPMC_E_UNSUPPORTED, "U-turn memory does not fit in LDS");
   const int w = waves_for_lds(per_wave);

@@ -125,9 +125,9 @@ struct Target {
   int kind, dim, n_obs;
   T p0, p1;
   double log_norm;
-  const T* __restrict__ x;   // [n_obs, dim_x] row-major (GLM)
-  const T* __restrict__ xt;  // [dim_x, n_obs] (GLM)
-  const T* __restrict__ y;
+  gptr<T> x;   // [n_obs, dim_x] row-major (GLM)
+  gptr<T> xt;  // [dim_x, n_obs] (GLM)
+  gptr<T> y;
   int x_pitch, xt_pitch;  // row pitches of X and X^T (elements)
   // per-lane cached data in the state layout
   T a[K], b[K];
@@ -141,12 +141,12 @@ struct Target {
   int bslot, bcode;
   bool sp0;  // the only bijector is Softplus, all of it in slot 0
   T inv_s0sq, inv_s0, inv_s1, s1sq;  // prior constants (hier)
-  T* lds;  // per-wave scratch (target_scratch_elems)
+  lptr<T> lds;  // per-wave scratch (target_scratch_elems)
   // GLMs: this wave's index among the waves sharing the chain, their count,
   // and the LDS buffer where their partial sums meet (two halves, used in
   // turn: a wave can only reuse a half after every wave has read it)
   int split_w = 0, split_n = 1, red_half = 0;
-  unsigned char* red = nullptr;
+  lptr<unsigned char> red = nullptr;
 
   __device__ void load(const hipmc_problem& pb, T* wave_lds) {
     const int lane = lane_id();
@@ -156,9 +156,9 @@ struct Target {
     p0 = (T)pb.param[0];
     p1 = (T)pb.param[1];
     log_norm = pb.log_norm;
-    x = (const T*)pb.x;
-    xt = (const T*)pb.xt;
-    y = (const T*)pb.y;
+    x = (gptr<T>)pb.x;
+    xt = (gptr<T>)pb.xt;
+    y = (gptr<T>)pb.y;
     x_pitch = pb.x_pitch;
     xt_pitch = pb.xt_pitch;
     const T* aux = (const T*)pb.aux;
@@ -188,7 +188,7 @@ struct Target {
     inv_s0sq = inv_s0 * inv_s0;
     inv_s1 = T(1) / p1;
     s1sq = p1 * p1;
-    lds = wave_lds;
+    lds = (lptr<T>)wave_lds;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int e = k * 64 + lane;
@@ -211,7 +211,7 @@ struct Target {
   __device__ void set_split(int w, int n, unsigned char* buf) {
     split_w = w;
     split_n = n;
-    red = buf;
+    red = (lptr<unsigned char>)buf;
   }
 
   // Per-lane partial log-prob (double) and wave-uniform part `lpu`; writes
@@ -425,7 +425,8 @@ struct Target {
   //   blocks past the row pitch are clamped to the row start (their
   //   accumulators are never read).
   template <int NBB, bool LINEAR>
-  __device__ void glm_rows(const T* beta, T* rbuf, int dx, int xp, int tp,
+  __device__ void glm_rows(lptr<const T> beta, lptr<T> rbuf, int dx, int xp,
+                           int tp,
                            T inv_s2, const int (&cst)[kGlmBlocks<T, K>],
                            typename Vec16<T>::type (&acc)[kGlmBlocks<T, K>],
                            double& lpl, T& ssq) {
@@ -440,7 +441,7 @@ struct Target {
     const int dlast = dx - 1;
     for (int n0 = split_w * 64 * V; n0 < n; n0 += split_n * 64 * V) {
       const int rr = n0 + V * lane;
-      const T* col = xt + (rr < tp ? rr : tp - V);
+      const gptr<T> col = xt + (rr < tp ? rr : tp - V);
       T yv[V];
 #pragma unroll
       for (int v = 0; v < V; ++v) {
@@ -450,7 +451,7 @@ struct Target {
       VT cur[SB];
 #pragma unroll
       for (int u = 0; u < SB; ++u)
-        cur[u] = *(const VT*)(col + (size_t)(u < dlast ? u : dlast) * tp);
+        cur[u] = *(gptr<VT>)(col + (size_t)(u < dlast ? u : dlast) * tp);
       VT s0 = VT{}, s1 = VT{};
       for (int bi = 0; bi + 1 < nsb; ++bi) {
         const int d = bi * SB;
@@ -458,7 +459,7 @@ struct Target {
 #pragma unroll
         for (int u = 0; u < SB; ++u) {
           const int dd = d + SB + u;
-          nxt[u] = *(const VT*)(col + (size_t)(dd < dlast ? dd : dlast) * tp);
+          nxt[u] = *(gptr<VT>)(col + (size_t)(dd < dlast ? dd : dlast) * tp);
         }
 #pragma unroll
         for (int u = 0; u < SB; u += 2) {
@@ -478,7 +479,7 @@ struct Target {
       }
       const VT sv = s0 + s1;
       const int rows = (n - n0) < 64 * V ? (n - n0) : 64 * V;
-      const T* xb = x + (size_t)n0 * xp;
+      const gptr<T> xb = x + (size_t)n0 * xp;
       VT gv[B][NBB];
 #pragma unroll
       for (int u = 0; u < B; ++u) {
@@ -486,7 +487,7 @@ struct Target {
         const int jl = jj < rows ? jj : rows - 1;
 #pragma unroll
         for (int b = 0; b < NBB; ++b)
-          gv[u][b] = *(const VT*)(xb + (size_t)jl * xp + cst[b]);
+          gv[u][b] = *(gptr<VT>)(xb + (size_t)jl * xp + cst[b]);
       }
 #pragma unroll
       for (int v = 0; v < V; ++v) {
@@ -521,7 +522,7 @@ struct Target {
           const int jl = jj < rows ? jj : rows - 1;
 #pragma unroll
           for (int b = 0; b < NBB; ++b)
-            nv[u][b] = *(const VT*)(xb + (size_t)jl * xp + cst[b]);
+            nv[u][b] = *(gptr<VT>)(xb + (size_t)jl * xp + cst[b]);
         }
 #pragma unroll
         for (int u = 0; u < B; ++u) {
@@ -566,7 +567,7 @@ struct Target {
   // meet beta = 0; blocks past the row pitch are clamped to the row start
   // (their accumulators are never read).
   template <int NBB, bool LINEAR>
-  __device__ void glm_pass(const T* beta, int dx, int xp, T inv_s2,
+  __device__ void glm_pass(lptr<const T> beta, int dx, int xp, T inv_s2,
                            const int (&cst)[kGlmBlocks<T, K>],
                            typename Vec16<T>::type (&acc)[kGlmBlocks<T, K>],
                            double& lpl, T& ssq) {
@@ -600,7 +601,7 @@ struct Target {
         const int jl = j < n ? j : n - 1;
 #pragma unroll
         for (int b = 0; b < NBB; ++b)
-          xv[u][b] = *(const VT*)(x + (size_t)jl * xp + cst[b]);
+          xv[u][b] = *(gptr<VT>)(x + (size_t)jl * xp + cst[b]);
       }
 #pragma unroll
       for (int k = 0; k < RPL; ++k) {
@@ -705,9 +706,9 @@ struct Target {
       return 0.0;
     }
     const int xp = x_pitch;
-    T* beta = lds;                 // [K*64]
-    T* rbuf = lds + K * 64;        // [64V] residuals of a chunk (narrow)
-    T* part = rbuf + 64 * V;       // [64 lanes][NB][V] gradient partials
+    lptr<T> beta = lds;            // [K*64]
+    lptr<T> rbuf = lds + K * 64;   // [64V] residuals of a chunk (narrow)
+    lptr<T> part = rbuf + 64 * V;  // [64 lanes][NB][V] gradient partials
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       const int e = k * 64 + lane;
@@ -769,10 +770,11 @@ struct Target {
     if (split_n > 1) {
       // every wave adds the waves' partials in the same order, so all of
       // them continue with identical sums
-      unsigned char* half = red + (size_t)red_half * glm_red_bytes<T, K>(split_n);
-      T* rg = (T*)half;
-      double* rl = (double*)(rg + (size_t)split_n * K * 64);
-      T* rs = (T*)(rl + (size_t)split_n * 64);
+      lptr<unsigned char> half =
+          red + (size_t)red_half * glm_red_bytes<T, K>(split_n);
+      lptr<T> rg = (lptr<T>)half;
+      lptr<double> rl = (lptr<double>)(rg + (size_t)split_n * K * 64);
+      lptr<T> rs = (lptr<T>)(rl + (size_t)split_n * 64);
 #pragma unroll
       for (int k = 0; k < K; ++k) rg[(split_w * K + k) * 64 + lane] = gl[k];
       rl[split_w * 64 + lane] = lpl;
