@@ -43,6 +43,10 @@
  *                        logistic regression at large batch: the
  *                        autodiff gradient of the Bernoulli-logits GLM
  *                        (bernoulli.py:119-135) as fused BF16 MFMA GEMMs.
+ *   hipmc_variance_update
+ *                        RunningVariance.update inside
+ *                        DiagonalMassMatrixAdaptation.one_step
+ *                        (experimental/mcmc/diagonal_mass_matrix_adaptation.py:186-228).
  *   hipmc_draws          the momentum / uniform draws of samplers.normal /
  *                        samplers.uniform (internal/samplers.py:308-367) under
  *                        this library's counter-keyed Philox schedule.
@@ -82,6 +86,10 @@ extern "C" {
 #define HIPMC_BIJ_EXP 2
 
 #define HIPMC_MAX_BIJ_RANGES 4
+
+/* reductions of the chains' log accept probabilities (hipmc_da_update) */
+#define HIPMC_REDUCE_MEAN 0
+#define HIPMC_REDUCE_HARMONIC 1
 
 /*
  * One posterior ("problem").  Lives in DEVICE memory as an array; the
@@ -334,7 +342,11 @@ int hipmc_target_eval(int32_t dtype, int32_t target_kind,
  * log_averaging_step, log_shrinkage_target, target_accept_prob.  `step` is
  * the DA step count before this update.  Writes step_size (dtype
  * [n_steps]) and, if non-NULL, new_step_size_f64 (which also supplies the
- * previous step size when set).
+ * previous step size when set).  reduce_mode: HIPMC_REDUCE_MEAN reduces the
+ * chains' log accept probabilities with reduce_logmeanexp (the class
+ * default), HIPMC_REDUCE_HARMONIC with reduce_log_harmonic_mean_exp
+ * (math/generic.py:387-432), as windowed adaptation configures it
+ * (experimental/mcmc/windowed_sampling.py:869-873).
  */
 int hipmc_da_update(int32_t dtype, int32_t n_steps, const int32_t* step_group,
                     const int32_t* state_index,
@@ -345,8 +357,8 @@ int hipmc_da_update(int32_t dtype, int32_t n_steps, const int32_t* step_group,
                     const double* target_accept_prob, int64_t step,
                     int64_t num_adaptation_steps, double exploration_shrinkage,
                     double step_count_smoothing, double decay_rate,
-                    void* step_size, double* new_step_size_f64,
-                    uint64_t stream);
+                    int32_t reduce_mode, void* step_size,
+                    double* new_step_size_f64, uint64_t stream);
 
 /*
  * SimpleStepSizeAdaptation update of n_elems step-size elements over n_rows
@@ -395,6 +407,24 @@ int hipmc_moments_update(int32_t dtype, int32_t mode, int64_t n_batch,
                          double* mean_out, double* ssr_out, uint64_t stream);
 
 /*
+ * Running variance of groups of chains: group g (chains
+ * [group_start[g], group_start[g+1]), row chain_row[c] of x, or c when
+ * chain_row is NULL) adds its chains' states x [rows, dim_stride] (dtype,
+ * first `dim` elements) as one chunk of observations to (num_samples[g],
+ * mean[g, :], ssr[g, :]) (float64; mean and ssr [n_groups, dim_stride]):
+ * chunk mean and residual sum in float64, then Pebay's merge.  Replaces the
+ * RunningVariance.update of DiagonalMassMatrixAdaptation.one_step
+ * (experimental/mcmc/diagonal_mass_matrix_adaptation.py:186-228), whose
+ * variance ssr / num_samples becomes the momentum precision at the end of
+ * each slow window.
+ */
+int hipmc_variance_update(int32_t dtype, int32_t n_groups,
+                          const int32_t* group_start, const int32_t* chain_row,
+                          const void* x, int32_t dim_stride, int32_t dim,
+                          double* num_samples, double* mean, double* ssr,
+                          uint64_t stream);
+
+/*
  * Metropolis-Hastings accept for n_chains chains:
  *   lar = safe_sum(proposed_tlp - current_tlp [+ correction]);
  *   accept = log(U) < lar, U from the MH purpose keyed by `seed` (counter
@@ -426,7 +456,12 @@ int hipmc_bijector(int32_t dtype, const hipmc_problem* problems,
  * stream on the host side.  kind 0: momentum [n_chains, dim];
  * kind 1: MH uniform [n_chains]; kind 2: NUTS tree draws for `depth`
  * ([n_chains] direction as 0/1 in out_aux (uint8) and uniform in out);
- * kind 3: NUTS leaf uniform for (`depth`, `index`) [n_chains].
+ * kind 3: NUTS leaf uniform for (`depth`, `index`) [n_chains];
+ * kind 4: initial positions uniform on (-2, 2) [n_chains, dim] (the role of
+ * init_near_unconstrained_zero, experimental/mcmc/initialization.py, for
+ * windowed adaptation): element e of chain c is
+ * 4 (w / 2^8 + 0.5) / 2^24 - 2 with w word e % 4 of the block
+ * (e / 4, c, problem, 'INIT').
  */
 int hipmc_draws(int32_t dtype, int32_t kind, const uint32_t seed[2],
                 int32_t problem, int32_t n_chains, int32_t dim, int32_t depth,

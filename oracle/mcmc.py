@@ -65,11 +65,14 @@ def reduce_logmeanexp(x, axis=None, keepdims=False):
 # leapfrog  (`mcmc/internal/leapfrog_integrator.py:222-362`)
 
 
-def leapfrog(target, eps, num_steps, p, q, t, g):
+def leapfrog(target, eps, num_steps, p, q, t, g, prec=None):
   """Returns (p', q', t', g') after `num_steps` leapfrog steps.
 
-  Op order as in the reference: p += (0.5*eps)*g; L x [q += eps*p;
-  (t, g) = value_and_grad(q); p += eps*g]; p -= (0.5*eps)*g.  `eps` is a
+  Op order as in the reference: p += (0.5*eps)*g; L x [q += eps*v(p);
+  (t, g) = value_and_grad(q); p += eps*g]; p -= (0.5*eps)*g, with the
+  velocity v(p) = grad of the kinetic energy (`_one_step`, `:319-362`):
+  p itself for the identity metric, prec * p for a diagonal momentum
+  precision `prec` ([D] or [C, D]; the preconditioned kernels).  `eps` is a
   scalar or per-chain [C] array.
   """
   eps = np.asarray(eps, np.float64)
@@ -78,7 +81,7 @@ def leapfrog(target, eps, num_steps, p, q, t, g):
   half = 0.5 * eps
   p = p + half * g
   for _ in range(num_steps):
-    q = q + eps * p
+    q = q + eps * (p if prec is None else prec * p)
     with np.errstate(all='ignore'):
       t, g = target.log_prob_and_grad(q)
     p = p + eps * g
@@ -97,21 +100,29 @@ HMCResults = collections.namedtuple('HMCResults', [
 
 
 def hmc_one_step(target, step_size, num_leapfrog_steps, q, t, g, seed,
-                 draws=None):
+                 draws=None, prec=None):
   """One calibrated HMC step for all chains.
 
   Seed use: MH splits `seed` into (proposal, acceptance)
   (`metropolis_hastings.py:183`); the proposal seed keys the momentum
   (`hmc.py:684-695`) and the acceptance seed keys the uniform
-  (`metropolis_hastings.py:221-226`).
+  (`metropolis_hastings.py:221-226`).  With a diagonal momentum precision
+  `prec` this is PreconditionedHamiltonianMonteCarlo
+  (`experimental/mcmc/preconditioned_hmc.py`): momentum z / sqrt(prec),
+  velocity prec * p, and the correction is the difference of the kinetic
+  energies 0.5 sum(prec p^2) (their normalising terms cancel).
   """
   draws = draws or orng.PhiloxDraws()
   c, d = q.shape
   prop_seed, acc_seed = orng.split_seed(seed, 2)
   p0 = draws.momentum(prop_seed, c, d).astype(np.float64)
-  p1, q1, t1, g1 = leapfrog(target, step_size, num_leapfrog_steps, p0, q, t, g)
+  if prec is not None:
+    p0 = p0 / np.sqrt(prec)
+  p1, q1, t1, g1 = leapfrog(target, step_size, num_leapfrog_steps, p0, q, t, g,
+                            prec)
+  m = 1.0 if prec is None else prec
   with np.errstate(all='ignore'):
-    corr = 0.5 * safe_sum([np.sum(p0 ** 2, -1), -np.sum(p1 ** 2, -1)])
+    corr = 0.5 * safe_sum([np.sum(m * p0 ** 2, -1), -np.sum(m * p1 ** 2, -1)])
     lar = safe_sum([t1, -t, corr])
     u = draws.mh_uniform(acc_seed, c).astype(np.float64)
     is_acc = np.log(u) < lar
@@ -209,16 +220,33 @@ NUTSResults = collections.namedtuple('NUTSResults', [
     'energy', 'tree_depth'])
 
 
-def compute_hamiltonian(t, p):
-  """`nuts.py:1085-1102`: returns tlp - 0.5*sum(p^2) (i.e. -H)."""
-  return t - 0.5 * np.sum(p ** 2, axis=-1)
+def compute_hamiltonian(t, p, prec=None):
+  """`nuts.py:1085-1102`: returns tlp - 0.5*sum(p^2) (i.e. -H).
+
+  With a diagonal momentum precision (`preconditioned_nuts.py:1091-1099`)
+  the kinetic energy is -log N(p; 0, diag(1/prec)) =
+  0.5 sum(prec p^2) + 0.5 d log(2 pi) - 0.5 sum(log prec).
+  """
+  if prec is None:
+    return t - 0.5 * np.sum(p ** 2, axis=-1)
+  d = p.shape[-1]
+  kin = (0.5 * np.sum(prec * p ** 2, axis=-1) + 0.5 * d * np.log(2 * np.pi)
+         - 0.5 * np.sum(np.log(prec) * np.ones_like(p), axis=-1))
+  return t - kin
 
 
 def nuts_one_step(target, step_size, q, t, g, seed, draws=None, max_depth=10,
-                  max_energy_diff=1000.0, unrolled_leapfrog_steps=1):
+                  max_energy_diff=1000.0, unrolled_leapfrog_steps=1,
+                  precision=None):
   """One NUTS transition, batched lockstep restatement of `nuts.py:321-946`.
 
-  `step_size` is a scalar or per-chain [C].  Returns (next_state, results).
+  `step_size` is a scalar or per-chain [C].  With `precision` ([D] or
+  [C, D], the diagonal precision of the momentum distribution) this is the
+  PreconditionedNoUTurnSampler (`experimental/mcmc/preconditioned_nuts.py:
+  330-1010`): momentum z / sqrt(prec), positions move with the velocity
+  prec * p, the U-turn memory holds velocities and the generalized U-turn
+  checks take momentum sums against velocities.  Returns (next_state,
+  results).
   """
   draws = draws or orng.PhiloxDraws()
   q = np.asarray(q, np.float64)
@@ -227,10 +255,16 @@ def nuts_one_step(target, step_size, q, t, g, seed, draws=None, max_depth=10,
   write, read = nuts_instructions(max_depth)
   mem = int(write.max()) + 1
   neg_inf = -np.inf
+  prec = None if precision is None else np.broadcast_to(
+      np.asarray(precision, np.float64), (c, d))
+  vel = (lambda p: p) if prec is None else (lambda p: prec * p)
 
-  # _start_trajectory_batched (`nuts.py:512-539`)
+  # _start_trajectory_batched (`nuts.py:512-539`,
+  # `preconditioned_nuts.py:525-545`)
   p0 = draws.momentum(seed, c, d).astype(np.float64)
-  h0 = compute_hamiltonian(t, p0)
+  if prec is not None:
+    p0 = p0 / np.sqrt(prec)
+  h0 = compute_hamiltonian(t, p0, prec)
 
   ends = {  # [2, C, ...]: 0 = left, 1 = right  (`nuts.py:342-350`)
       'p': np.stack([p0, p0]), 'q': np.stack([q, q]),
@@ -269,7 +303,8 @@ def nuts_one_step(target, step_size, q, t, g, seed, draws=None, max_depth=10,
     while i < (1 << depth) and c_sub.any():
       # _loop_build_sub_tree (`nuts.py:793-946`)
       p1, q1, t1, g1 = leapfrog(target, eps, unrolled_leapfrog_steps,
-                                fr['p'], fr['q'], fr['t'], fr['g'])
+                                fr['p'], fr['q'], fr['t'], fr['g'], prec)
+      v1 = vel(p1)
       fr = {'p': p1, 'q': q1, 't': t1, 'g': g1}
       cs_prev = cs
       cs = cs + p1
@@ -279,13 +314,13 @@ def nuts_one_step(target, step_size, q, t, g, seed, draws=None, max_depth=10,
       while j < read[i, 1] and no_u.any():
         diff = cs - mem_c[j]
         dl = np.sum(diff * mem_p[j], -1)
-        dr = np.sum(diff * p1, -1)
+        dr = np.sum(diff * v1, -1)
         no_u = no_u & (dl >= 0) & (dr >= 0)
         j += 1
-      mem_p[write[i]] = p1
+      mem_p[write[i]] = v1
       mem_c[write[i]] = cs_prev
       with np.errstate(invalid='ignore'):
-        energy = compute_hamiltonian(t1, p1)
+        energy = compute_hamiltonian(t1, p1, prec)
       energy = np.where(np.isnan(energy), neg_inf, energy)
       with np.errstate(invalid='ignore'):
         de = energy - h0
@@ -322,8 +357,8 @@ def nuts_one_step(target, step_size, q, t, g, seed, draws=None, max_depth=10,
     ends = {k: np.stack([sel(other[k], fr[k]), sel(fr[k], other[k])])
             for k in ends}
     msum = msum + cs
-    no_u_traj = ((np.sum(msum * ends['p'][0], -1) >= 0)
-                 & (np.sum(msum * ends['p'][1], -1) >= 0))
+    no_u_traj = ((np.sum(msum * vel(ends['p'][0]), -1) >= 0)
+                 & (np.sum(msum * vel(ends['p'][1]), -1) >= 0))
     is_acc = is_acc | choose
     cont = c_sub & no_u_traj
     notdiv = nd_sub
@@ -411,15 +446,15 @@ def da_update(state, log_accept_ratio, state_rank, num_adaptation_steps,
 def da_update_grouped(state, log_accept_ratio, groups, n_groups,
                       num_adaptation_steps, target_accept_prob=0.75,
                       exploration_shrinkage=0.05, step_count_smoothing=10.0,
-                      decay_rate=0.75):
+                      decay_rate=0.75, reduce_fn=reduce_logmeanexp):
   """DA for step sizes of shape [n_groups] where chain c belongs to groups[c].
 
-  Same arithmetic as `da_update`; the log-mean-exp runs over each group's
-  chains (one problem's chains in a batched launch).
+  Same arithmetic as `da_update`; the reduction (log-mean-exp by default)
+  runs over each group's chains (one problem's chains in a batched launch).
   """
   lap = log_accept_prob(log_accept_ratio)
   groups = np.asarray(groups)
-  red = np.array([reduce_logmeanexp(lap[groups == k]) for k in range(n_groups)])
+  red = np.array([reduce_fn(lap[groups == k]) for k in range(n_groups)])
   err = 0.0 if state.error_sum is None else state.error_sum
   new_err = err + np.asarray(target_accept_prob) - np.exp(red)
   t = float(state.step) + 1.0
@@ -529,3 +564,176 @@ def sample_nuts_da(target, init_x, step_size, num_adaptation_steps,
       out_x.append(x)
       trace.append(r)
   return np.stack(out_x), trace, da
+
+
+# --------------------------------------------------------------------------
+# Windowed adaptation  (`experimental/mcmc/windowed_sampling.py:329-500,
+#                       603-961`, `diagonal_mass_matrix_adaptation.py`)
+
+
+def reduce_log_harmonic_mean_exp(x, axis=None, keepdims=False):
+  """`math/generic.py:387-432`: -reduce_logmeanexp(-x)."""
+  return -reduce_logmeanexp(-np.asarray(x, np.float64), axis=axis,
+                            keepdims=keepdims)
+
+
+def window_sizes(num_adaptation_steps):
+  """`_get_window_sizes` (`windowed_sampling.py:367-395`): first fast
+  window, first slow window, last fast window."""
+  slow = num_adaptation_steps // 20
+  first = 3 * slow
+  last = num_adaptation_steps - 15 * slow - first
+  return first, slow, last
+
+
+def window_option(step, num_adaptation_steps):
+  """Which update `WindowedAdaptation.one_step` applies before the inner
+  step `step` (`windowed_sampling.py:471-494`): 0 none, 1 first fast window
+  start, 2 first slow window start, 3 later slow window start, 4 last
+  window start; coinciding starts add up and the sum is clipped to 4
+  (`mcmc_util.choose_from`, `mcmc/internal/util.py:176-195`)."""
+  first, slow, _ = window_sizes(num_adaptation_steps)
+  opt = int(step == 0)
+  opt += 2 * int(step == first)
+  opt += 3 * int(any(step == first + slow * (2 ** i - 1) for i in (1, 2, 3)))
+  opt += 4 * int(step == first + (2 ** 4 - 1) * slow)
+  return min(opt, 4)
+
+
+WindowedState = collections.namedtuple('WindowedState', [
+    'da', 'da_num_adaptation_steps', 'running', 'dmma_step',
+    'num_estimation_steps', 'precision'])
+
+
+def windowed_bootstrap(step_size, num_adaptation_steps, dim, n_groups=1):
+  """Bootstrap of WindowedAdaptation(DiagonalMassMatrixAdaptation(
+  DualAveraging(...))) for step sizes [n_groups] and precisions
+  [n_groups, dim]: unit running variance with zero samples
+  (`windowed_sampling.py:908-913`), DMMA estimating over
+  num_adaptation_steps (`:323-327`), identity momentum
+  (`_init_momentum`, `:578-585`)."""
+  from oracle import reducers
+  step = np.broadcast_to(np.asarray(step_size, np.float64), (n_groups,))
+  running = reducers.Moments((n_groups, dim), 1, 1)
+  return WindowedState(
+      da=da_bootstrap(step.copy()),
+      da_num_adaptation_steps=num_adaptation_steps, running=running,
+      dmma_step=0, num_estimation_steps=num_adaptation_steps,
+      precision=np.ones((n_groups, dim)))
+
+
+def windowed_pre_step(ws, step, num_adaptation_steps, dim, n_groups=1):
+  """The window update of `WindowedAdaptation.one_step` before inner step
+  `step` (`windowed_sampling.py:415-469`): resets of the dual averaging
+  (keeping its current step size, `_bootstrap_from_inner_results`,
+  `dual_averaging_step_size_adaptation.py:543-610`) and of the running
+  variance, and the window lengths they run for."""
+  from oracle import reducers
+  first, slow, last = window_sizes(num_adaptation_steps)
+  opt = window_option(step, num_adaptation_steps)
+  if opt == 1:
+    return ws._replace(da_num_adaptation_steps=first + slow,
+                       num_estimation_steps=-1)
+  if opt == 2:
+    return ws._replace(dmma_step=0, num_estimation_steps=slow)
+  if opt in (3, 4):
+    da = da_bootstrap(np.asarray(ws.da.step_size, np.float64))
+    if opt == 3:
+      cur = step - first + slow
+      return ws._replace(da=da, da_num_adaptation_steps=cur,
+                         running=reducers.Moments((n_groups, dim), 1, 1),
+                         dmma_step=0, num_estimation_steps=cur)
+    return ws._replace(da=da, da_num_adaptation_steps=last)
+  return ws
+
+
+def windowed_post_step(ws, x, log_accept_ratio, groups, n_groups,
+                       target_accept_prob):
+  """After the inner step: dual averaging with the harmonic-mean reduction
+  over each group's chains (`windowed_sampling.py:869-873`), then the
+  DiagonalMassMatrixAdaptation bookkeeping
+  (`diagonal_mass_matrix_adaptation.py:186-244`): the chains' new states
+  enter the group's running variance while step <= num_estimation_steps,
+  and at step == num_estimation_steps the momentum precision becomes that
+  variance (population variance, ddof 0)."""
+  from oracle import reducers
+  da = da_update_grouped(ws.da, log_accept_ratio, groups, n_groups,
+                         ws.da_num_adaptation_steps, target_accept_prob,
+                         reduce_fn=reduce_log_harmonic_mean_exp)
+  step = ws.dmma_step + 1
+  running, prec = ws.running, ws.precision
+  if step <= ws.num_estimation_steps:
+    running = _grouped_moments_update(running, x, groups, n_groups)
+  if step == ws.num_estimation_steps:
+    prec = reducers.covariance(running)
+  return ws._replace(da=da, running=running, dmma_step=step, precision=prec)
+
+
+def _grouped_moments_update(m, x, groups, n_groups):
+  """RunningVariance.update of each group by its chains' states
+  (`diagonal_mass_matrix_adaptation.py:200-228`: the chain dimensions are
+  flattened into one chunk of observations)."""
+  from oracle import reducers
+  out = reducers.Moments(m.mean.shape, 1, 1)
+  nums_in = np.broadcast_to(np.asarray(m.num_samples, np.float64).reshape(-1),
+                            (n_groups,))
+  means, ssrs, nums = [], [], []
+  for k in range(n_groups):
+    sub = reducers.Moments((x.shape[1],), 1, 1)
+    sub.num_samples = float(nums_in[k])
+    sub.mean = m.mean[k].copy()
+    sub.ssr = m.ssr[k].copy()
+    sub = reducers.update(sub, x[groups == k], axis=0)
+    means.append(sub.mean)
+    ssrs.append(sub.ssr)
+    nums.append(sub.num_samples)
+  out.mean = np.stack(means)
+  out.ssr = np.stack(ssrs)
+  out.num_samples = np.asarray(nums, np.float64)[:, None]
+  return out
+
+
+def sample_windowed(target, init_x, step_size, num_adaptation_steps,
+                    num_results, seed, kind='nuts', target_accept_prob=None,
+                    max_depth=10, num_leapfrog_steps=10, dtype=np.float64,
+                    groups=None, n_groups=1, problem=0):
+  """windowed_adaptive_nuts / windowed_adaptive_hmc
+  (`windowed_sampling.py:603-961`) on an unconstrained target, discarding
+  the tuning draws: sample_chain of WindowedAdaptation(
+  DiagonalMassMatrixAdaptation(DualAveraging(Preconditioned{NUTS,HMC})))
+  with num_burnin_steps = num_adaptation_steps.  Returns the states
+  [R, C, D], the per-step inner results, the final windowed state and a
+  dict with every step's state [S, C, D] and the precision [S, G, D] each
+  step ran with."""
+  if target_accept_prob is None:
+    target_accept_prob = 0.85 if kind == 'nuts' else 0.75
+  draws = orng.PhiloxDraws(dtype, problem=problem)
+  x = np.asarray(init_x, np.float64)
+  c, d = x.shape
+  groups = np.zeros(c, np.int64) if groups is None else np.asarray(groups)
+  t, g = target.log_prob_and_grad(x)
+  ws = windowed_bootstrap(step_size, num_adaptation_steps, d, n_groups)
+  n_total = num_adaptation_steps + num_results
+  seeds = step_seeds(seed, n_total)
+  out_x, trace, states, precs = [], [], [], []
+  for s in range(n_total):
+    ws = windowed_pre_step(ws, s, num_adaptation_steps, d, n_groups)
+    precs.append(ws.precision)
+    eps = np.asarray(ws.da.step_size)[groups]
+    prec = ws.precision[groups]
+    if kind == 'nuts':
+      x, r = nuts_one_step(target, eps, x, t, g, seeds[s], draws,
+                           max_depth=max_depth, precision=prec)
+    else:
+      x, r = hmc_one_step(target, eps, num_leapfrog_steps, x, t, g, seeds[s],
+                          draws, prec=prec)
+    t, g = r.target_log_prob, r.grads_target_log_prob
+    ws = windowed_post_step(ws, x, r.log_accept_ratio, groups, n_groups,
+                            target_accept_prob)
+    if s >= num_adaptation_steps:
+      out_x.append(x)
+    trace.append(r)
+    states.append(x)
+  info = {'states': np.stack(states), 'precision_used': np.stack(precs)}
+  return ((np.stack(out_x) if out_x else np.zeros((0, c, d))), trace, ws,
+          info)

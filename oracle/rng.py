@@ -132,6 +132,22 @@ def momentum(seed, chains, dim, dtype=np.float64, problem=0):
   return out[:, :dim].astype(dtype)
 
 
+PURPOSE_INIT = 0x494E4954
+
+
+def init_uniform(seed, chains, dim, dtype=np.float64, problem=0):
+  """Initial positions uniform on (-2, 2) [len(chains), dim]: element e
+  from word e % 4 of block (e // 4, chain, problem, INIT), 24 bits, centred
+  in its bin (4 (w / 2^8 + 0.5) / 2^24 - 2)."""
+  chains = np.asarray(chains, dtype=np.uint64)
+  nblk = (dim + 3) // 4
+  blk = np.arange(nblk, dtype=np.uint64)
+  w = _block(seed, blk[None, :], chains[:, None], problem, PURPOSE_INIT)
+  w = w.reshape(len(chains), nblk * 4)[:, :dim]
+  u = ((w >> np.uint32(8)).astype(np.float64) + 0.5) * _TWO_M24
+  return (4.0 * u - 2.0).astype(dtype)
+
+
 def mh_uniform(seed, chains, dtype=np.float64, problem=0):
   w = _block(seed, 0, np.asarray(chains, np.uint64), problem, PURPOSE_MH_ACCEPT)
   return uniform_from_words(w, dtype)

@@ -20,6 +20,7 @@ BIJ_IDENTITY, BIJ_SOFTPLUS, BIJ_EXP = 0, 1, 2
 MAX_BIJ_RANGES = 4
 
 E_INVALID, E_UNSUPPORTED, E_HIP = -1, -2, -3
+REDUCE_MEAN, REDUCE_HARMONIC = 0, 1
 
 EXPORTED_SYMBOLS = (
     'hipmc_last_error', 'hipmc_abi_version', 'hipmc_sizeof_problem',
@@ -28,7 +29,8 @@ EXPORTED_SYMBOLS = (
     'hipmc_seed_salt', 'hipmc_philox_host', 'hipmc_hmc_run',
     'hipmc_leapfrog_run', 'hipmc_nuts_run', 'hipmc_target_eval',
     'hipmc_da_update', 'hipmc_simple_step_update', 'hipmc_mh_accept',
-    'hipmc_bijector', 'hipmc_draws', 'hipmc_moments_update')
+    'hipmc_bijector', 'hipmc_draws', 'hipmc_moments_update',
+    'hipmc_variance_update')
 
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32
@@ -123,9 +125,11 @@ def lib():
       l.hipmc_glm_hmc_run.argtypes = [ctypes.POINTER(GlmHmcArgs), _u64]
       l.hipmc_target_eval.argtypes = [_i32, _i32, _vp, _vp, _i32, _i32, _i32,
                                       _vp, _vp, _vp, _u64]
-      l.hipmc_da_update.argtypes = [_i32, _i32, _vp, _vp, _vp, _vp, _i32, _i64, _vp, _vp, _vp, _vp,
-                                    _i64, _i64, _f64, _f64, _f64, _vp, _vp,
-                                    _u64]
+      l.hipmc_da_update.argtypes = [_i32, _i32, _vp, _vp, _vp, _vp, _i32,
+                                    _i64, _vp, _vp, _vp, _vp, _i64, _i64,
+                                    _f64, _f64, _f64, _i32, _vp, _vp, _u64]
+      l.hipmc_variance_update.argtypes = [_i32, _i32, _vp, _vp, _vp, _i32,
+                                          _i32, _vp, _vp, _vp, _u64]
       l.hipmc_simple_step_update.argtypes = [_i32, _i32, _vp, _vp, _vp, _i32,
                                              _i64, _vp, _f64, _i64, _i64, _vp,
                                              _u64]

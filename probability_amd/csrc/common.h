@@ -38,6 +38,8 @@ constexpr uint32_t kPurposeMomentum = 0x4D4F4D00u;
 constexpr uint32_t kPurposeMHAccept = 0x4D484100u;
 constexpr uint32_t kPurposeTree = 0x54520000u;
 constexpr uint32_t kPurposeLeaf = 0x4C460000u;
+// initial positions of windowed adaptation (uniform on (-2, 2))
+constexpr uint32_t kPurposeInit = 0x494E4954u;
 
 struct Seed {
   uint32_t k0, k1;

@@ -126,14 +126,18 @@ struct EvalL {
 // reduce_logmeanexp of hmc_like_log_accept_prob_getter_fn(lar) over chains
 // [b, e) (`simple_step_size_adaptation.py:42-48`, `math/generic.py:221-273`),
 // float64; every thread of the block returns the same value.
+// harmonic = true: reduce_log_harmonic_mean_exp = -logmeanexp(-x)
+// (`math/generic.py:387-432`), the reduction windowed adaptation gives its
+// dual averaging (`experimental/mcmc/windowed_sampling.py:869-873`).
 template <typename T>
 __device__ double group_log_mean_accept(const T* lar, int b, int e,
-                                        double* red) {
+                                        double* red, bool harmonic = false) {
   const int tid = threadIdx.x;
+  const double sgn = harmonic ? -1.0 : 1.0;
   auto lap_of = [&](int i) -> double {
     double v = (double)lar[i];
     if (!__builtin_isfinite(v)) v = -__builtin_inf();
-    return fmin(v, 0.0);
+    return sgn * fmin(v, 0.0);
   };
   double m = -__builtin_inf();
   for (int i = b + tid; i < e; i += blockDim.x) m = fmax(m, lap_of(i));
@@ -156,7 +160,7 @@ __device__ double group_log_mean_accept(const T* lar, int b, int e,
   }
   const double total = red[0];
   __syncthreads();
-  return log(total) + mx - log((double)(e - b));
+  return sgn * (log(total) + mx - log((double)(e - b)));
 }
 
 // One block per step-size group; float64 throughout
@@ -169,7 +173,7 @@ __global__ void __launch_bounds__(256)
               double* error_sum, double* log_avg, const double* log_shrink,
               const double* target_accept, int64_t step0, int64_t num_adapt,
               double gamma, double t0, double kappa, T* step_size,
-              double* step_f64) {
+              double* step_f64, int harmonic) {
   __shared__ double red[256];
   const int elem = blockIdx.x;
   const int grp = step_group ? step_group[elem] : elem;
@@ -182,7 +186,7 @@ __global__ void __launch_bounds__(256)
   for (int r = 0; r < n_rows; ++r) {
     const T* lar = lar_base + (size_t)r * row_stride;
     // reduce_logmeanexp, then `_one_step_part`
-    const double rlap = group_log_mean_accept(lar, b, e, red);
+    const double rlap = group_log_mean_accept(lar, b, e, red, harmonic != 0);
     err = err + target_accept[gidx] - exp(rlap);
     const int64_t step = step0 + r;
     const double t = (double)step + 1.0;
@@ -315,6 +319,19 @@ __global__ void draws_kernel(int kind, Seed sd, uint32_t prob, int n, int dim,
     out[i] = (T)((e & 1) ? n1 : n0);
     return;
   }
+  if (kind == 4) {
+    // uniform on (-2, 2): element e of chain c from word e % 4 of block
+    // (e / 4, c, problem, INIT)
+    if (i >= (size_t)n * dim) return;
+    const uint32_t c = (uint32_t)(i / dim);
+    const int e = (int)(i % dim);
+    const U4 w = philox(U4{(uint32_t)(e >> 2), c, prob, kPurposeInit}, sd.k0,
+                        sd.k1);
+    const uint32_t wd = (e & 3) == 0 ? w.x : (e & 3) == 1 ? w.y
+                      : (e & 3) == 2 ? w.z : w.w;
+    out[i] = (T)(4.0 * ((double)(wd >> 8) + 0.5) * (1.0 / 16777216.0) - 2.0);
+    return;
+  }
   if (i >= (size_t)n) return;
   const uint32_t c = (uint32_t)i;
   if (kind == 1) {
@@ -441,6 +458,44 @@ __global__ void moments_cross_kernel(const T* __restrict__ x, int64_t n_batch,
                            adj * (mi - mean_in[ei]) * (mj - mean_in[ej]);
     }
   }
+}
+
+// Running variance of each group's states (the DiagonalMassMatrixAdaptation
+// update, experimental/mcmc/diagonal_mass_matrix_adaptation.py:186-228: the
+// chains of a group are one chunk of observations).  One block per group;
+// thread e takes element e (strided): the chunk's mean and residual sum in
+// float64, then Pebay's merge into (num_samples, mean, ssr).  num_samples
+// is read by every thread before thread 0 writes the new count.
+template <typename T>
+__global__ void __launch_bounds__(256)
+    variance_update_kernel(const int32_t* group_start, const int32_t* chain_row,
+                           const T* x, int dim_stride, int dim,
+                           double* num_samples, double* mean, double* ssr) {
+  const int g = blockIdx.x;
+  const int b = group_start[g], e1 = group_start[g + 1];
+  const double n0 = num_samples[g];
+  const double cn = (double)(e1 - b);
+  const double nn = n0 + cn;
+  for (int e = threadIdx.x; e < dim; e += blockDim.x) {
+    double s = 0.0;
+    for (int c = b; c < e1; ++c) {
+      const int r = chain_row ? chain_row[c] : c;
+      s += (double)x[(size_t)r * dim_stride + e];
+    }
+    const double cm = s / cn;
+    double cs = 0.0;
+    for (int c = b; c < e1; ++c) {
+      const int r = chain_row ? chain_row[c] : c;
+      const double d = (double)x[(size_t)r * dim_stride + e] - cm;
+      cs += d * d;
+    }
+    const size_t i = (size_t)g * dim_stride + e;
+    const double delta = cm - mean[i];
+    ssr[i] = ssr[i] + cs + (n0 * cn / nn) * delta * delta;
+    mean[i] = mean[i] + cn * delta / nn;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) num_samples[g] = nn;
 }
 
 static unsigned moments_grid(int64_t n) {
@@ -572,8 +627,10 @@ int hipmc_da_update(int32_t dtype, int32_t n_groups, const int32_t* step_group,
                     const double* target_accept_prob, int64_t step,
                     int64_t num_adaptation_steps, double exploration_shrinkage,
                     double step_count_smoothing, double decay_rate,
-                    void* step_size, double* new_step_size_f64,
-                    uint64_t stream) {
+                    int32_t reduce_mode, void* step_size,
+                    double* new_step_size_f64, uint64_t stream) {
+  if (reduce_mode != HIPMC_REDUCE_MEAN && reduce_mode != HIPMC_REDUCE_HARMONIC)
+    return fail(HIPMC_E_INVALID, "da_update: bad reduce_mode");
   if (n_groups < 1 || !group_start || !lar || !error_sum ||
       !log_averaging_step || !log_shrinkage_target || !target_accept_prob ||
       !step_size)
@@ -588,7 +645,7 @@ int hipmc_da_update(int32_t dtype, int32_t n_groups, const int32_t* step_group,
                        log_averaging_step, log_shrinkage_target,
                        target_accept_prob, step, num_adaptation_steps,
                        exploration_shrinkage, step_count_smoothing, decay_rate,
-                       (float*)step_size, new_step_size_f64);
+                       (float*)step_size, new_step_size_f64, reduce_mode);
   else if (dtype == HIPMC_F64)
     hipLaunchKernelGGL(da_kernel<double>, dim3(n_groups), dim3(256), 0, s,
                        step_group, state_index, group_start,
@@ -597,7 +654,7 @@ int hipmc_da_update(int32_t dtype, int32_t n_groups, const int32_t* step_group,
                        log_averaging_step, log_shrinkage_target,
                        target_accept_prob, step, num_adaptation_steps,
                        exploration_shrinkage, step_count_smoothing, decay_rate,
-                       (double*)step_size, new_step_size_f64);
+                       (double*)step_size, new_step_size_f64, reduce_mode);
   else
     return fail(HIPMC_E_INVALID, "bad dtype");
   return hip_check(hipGetLastError(), "da_kernel launch");
@@ -656,6 +713,32 @@ int hipmc_moments_update(int32_t dtype, int32_t mode, int64_t n_batch,
                           (const double*)sample, num_samples, mean_in, ssr_in,
                           mean_out, ssr_out, s);
   return fail(HIPMC_E_INVALID, "bad dtype");
+}
+
+int hipmc_variance_update(int32_t dtype, int32_t n_groups,
+                          const int32_t* group_start, const int32_t* chain_row,
+                          const void* x, int32_t dim_stride, int32_t dim,
+                          double* num_samples, double* mean, double* ssr,
+                          uint64_t stream) {
+  if (n_groups < 0 || !group_start || !x || !num_samples || !mean || !ssr)
+    return fail(HIPMC_E_INVALID, "variance_update: NULL argument");
+  if (dim < 0 || dim > dim_stride)
+    return fail(HIPMC_E_INVALID, "variance_update: need 0 <= dim <= dim_stride");
+  if (n_groups == 0) return HIPMC_OK;
+  hipStream_t s = as_stream(stream);
+  if (dtype == HIPMC_F32)
+    hipLaunchKernelGGL(variance_update_kernel<float>, dim3(n_groups),
+                       dim3(256), 0, s, group_start, chain_row,
+                       (const float*)x, dim_stride, dim, num_samples, mean,
+                       ssr);
+  else if (dtype == HIPMC_F64)
+    hipLaunchKernelGGL(variance_update_kernel<double>, dim3(n_groups),
+                       dim3(256), 0, s, group_start, chain_row,
+                       (const double*)x, dim_stride, dim, num_samples, mean,
+                       ssr);
+  else
+    return fail(HIPMC_E_INVALID, "bad dtype");
+  return hip_check(hipGetLastError(), "variance_update_kernel launch");
 }
 
 int hipmc_mh_accept(int32_t dtype, int32_t n, int32_t stride,
@@ -722,8 +805,8 @@ int hipmc_draws(int32_t dtype, int32_t kind, const uint32_t seed[2],
                 int32_t problem, int32_t n, int32_t dim, int32_t depth,
                 int32_t index, void* out, uint8_t* aux, uint64_t stream) {
   if (!seed || !out) return fail(HIPMC_E_INVALID, "draws: NULL argument");
-  if (kind < 0 || kind > 3) return fail(HIPMC_E_INVALID, "draws: bad kind");
-  const size_t ne = kind == 0 ? (size_t)n * dim : (size_t)n;
+  if (kind < 0 || kind > 4) return fail(HIPMC_E_INVALID, "draws: bad kind");
+  const size_t ne = (kind == 0 || kind == 4) ? (size_t)n * dim : (size_t)n;
   if (ne == 0) return HIPMC_OK;
   hipStream_t s = as_stream(stream);
   const Seed sd{seed[0], seed[1]};

@@ -36,7 +36,8 @@ __all__ = ['CovarianceReducer', 'ExpectationsReducer',
            'PotentialScaleReductionReducer', 'Reducer',
            'SampleDiscardingKernel', 'SampleDiscardingKernelResults',
            'VarianceReducer', 'WithReductions', 'WithReductionsKernelResults',
-           'sample_fold', 'step_kernel']
+           'default_hmc_trace_fn', 'default_nuts_trace_fn', 'sample_fold',
+           'step_kernel', 'windowed_adaptive_hmc', 'windowed_adaptive_nuts']
 
 
 # ------------------------------------------------------------- structures
@@ -369,3 +370,9 @@ def sample_fold(num_steps, current_state, previous_kernel_results=None,
   if return_final_reducer_states:
     return finalized, state, inner, reductions
   return finalized, state, inner
+
+
+# windowed adaptation (`experimental/mcmc/windowed_sampling.py`)
+from probability_amd.experimental.windowed_sampling import (  # noqa: E402
+    default_hmc_trace_fn, default_nuts_trace_fn, windowed_adaptive_hmc,
+    windowed_adaptive_nuts)

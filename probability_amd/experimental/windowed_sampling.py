@@ -1,0 +1,263 @@
+"""Windowed adaptation: `windowed_adaptive_nuts` and `windowed_adaptive_hmc`
+(`experimental/mcmc/windowed_sampling.py:603-961`).
+
+The reference samples a JointDistribution with the preconditioned NUTS or
+HMC kernel (`experimental/mcmc/preconditioned_nuts.py`,
+`preconditioned_hmc.py`) under DualAveragingStepSizeAdaptation (harmonic-mean
+reduction over chains, target 0.85 for NUTS and 0.75 for HMC) and
+DiagonalMassMatrixAdaptation (`diagonal_mass_matrix_adaptation.py`): the
+momentum's diagonal precision is the population variance of the states of
+each slow window (windows of s, 2s, 4s, 8s steps with s = A // 20 after a
+3s-step fast window, then a last fast window), and the dual averaging
+restarts at each window boundary from its current step size.
+
+Here the model is compiled to a device target spec
+(`probability_amd.distributions.compile_target`), the chains start uniform
+on (-2, 2) in the unconstrained space (the role of
+`init_near_unconstrained_zero`; keyed Philox draws of the setup seed), and
+the whole run executes on the GPU through the batched run loop
+(`probability_amd.mcmc.batched`): per adaptation step one sampler launch
+with the momentum precision, one dual-averaging launch and, inside slow
+windows, one running-variance launch; after adaptation the draws come from
+fused multi-iteration launches.  The window schedule is host bookkeeping.
+"""
+
+import collections
+
+import numpy as np
+import torch
+
+from probability_amd import _lib
+from probability_amd import distributions as dists
+from probability_amd import math as pmath
+from probability_amd import random as samplers
+from probability_amd.mcmc import batched
+from probability_amd.mcmc import sample as sample_lib
+
+__all__ = ['windowed_adaptive_nuts', 'windowed_adaptive_hmc',
+           'default_nuts_trace_fn', 'default_hmc_trace_fn',
+           'WindowedAdaptationResults']
+
+WindowedAdaptationResults = collections.namedtuple(
+    'WindowedAdaptationResults',
+    ['step_size', 'momentum_precision', 'running_variance_count',
+     'running_variance_mean', 'running_variance_ssr', 'step'])
+
+
+def default_nuts_trace_fn(state, bijector, is_adapting, pkr):
+  """Marker for the reference's default NUTS trace (`windowed_sampling.py:
+  61-98`): step_size, tune, target_log_prob, diverging, accept_ratio,
+  variance_scaling, n_steps, is_accepted.  The fields are assembled from the
+  device run's per-step outputs; the function is not called."""
+  raise NotImplementedError('traced on the device by windowed_adaptive_nuts')
+
+
+def default_hmc_trace_fn(state, bijector, is_adapting, pkr):
+  """Marker for the reference's default HMC trace (`windowed_sampling.py:
+  101-148`): step_size, tune, target_log_prob, diverging (|log accept
+  ratio| > 500), log_acceptance_correction, accept_ratio, variance_scaling,
+  is_accepted."""
+  raise NotImplementedError('traced on the device by windowed_adaptive_hmc')
+
+
+def windowed_adaptive_nuts(n_draws, joint_dist, *, n_chains=64,
+                           num_adaptation_steps=500, current_state=None,
+                           init_step_size=None, dual_averaging_kwargs=None,
+                           max_tree_depth=10, max_energy_diff=500.,
+                           unrolled_leapfrog_steps=1, parallel_iterations=10,
+                           trace_fn=default_nuts_trace_fn,
+                           return_final_kernel_results=False,
+                           discard_tuning=True, chain_axis_names=None,
+                           seed=None, **pins):
+  """Adapts step size and a diagonal mass matrix in expanding windows, then
+  draws `n_draws` NUTS samples (`windowed_sampling.py:603-729`).
+
+  Returns the draws in the model's structure (a list for
+  JointDistributionSequential, a dict for JointDistributionNamed) with
+  leading dimensions [n_draws (+ num_adaptation_steps), *n_chains]; with a
+  trace_fn, StatesAndTrace; with return_final_kernel_results,
+  CheckpointableStatesAndTrace.
+  """
+  del parallel_iterations
+  da = dict(dual_averaging_kwargs or {})
+  da.setdefault('target_accept_prob', 0.85)
+  return _windowed_impl(
+      'nuts', n_draws, joint_dist, n_chains=n_chains,
+      num_adaptation_steps=num_adaptation_steps, current_state=current_state,
+      init_step_size=init_step_size, dual_averaging_kwargs=da,
+      max_tree_depth=max_tree_depth, max_energy_diff=max_energy_diff,
+      unrolled_leapfrog_steps=unrolled_leapfrog_steps, num_leapfrog_steps=1,
+      trace_fn=trace_fn, default_trace=default_nuts_trace_fn,
+      return_final_kernel_results=return_final_kernel_results,
+      discard_tuning=discard_tuning, chain_axis_names=chain_axis_names,
+      seed=seed, pins=pins)
+
+
+def windowed_adaptive_hmc(n_draws, joint_dist, *, num_leapfrog_steps,
+                          n_chains=64, num_adaptation_steps=500,
+                          current_state=None, init_step_size=None,
+                          dual_averaging_kwargs=None,
+                          trace_fn=default_hmc_trace_fn,
+                          return_final_kernel_results=False,
+                          discard_tuning=True, chain_axis_names=None,
+                          seed=None, **pins):
+  """As `windowed_adaptive_nuts` with PreconditionedHamiltonianMonteCarlo of
+  `num_leapfrog_steps` (`windowed_sampling.py:732-843`)."""
+  da = dict(dual_averaging_kwargs or {})
+  da.setdefault('target_accept_prob', 0.75)
+  return _windowed_impl(
+      'hmc', n_draws, joint_dist, n_chains=n_chains,
+      num_adaptation_steps=num_adaptation_steps, current_state=current_state,
+      init_step_size=init_step_size, dual_averaging_kwargs=da,
+      max_tree_depth=10, max_energy_diff=1000.0, unrolled_leapfrog_steps=1,
+      num_leapfrog_steps=num_leapfrog_steps, trace_fn=trace_fn,
+      default_trace=default_hmc_trace_fn,
+      return_final_kernel_results=return_final_kernel_results,
+      discard_tuning=discard_tuning, chain_axis_names=chain_axis_names,
+      seed=seed, pins=pins)
+
+
+def _pin_dtype(pins):
+  for v in pins.values():
+    dt = getattr(v, 'dtype', None)
+    if dt in (torch.float64, np.float64):
+      return torch.float64
+  return torch.float32
+
+
+def init_uniform(batch, seed):
+  """Unconstrained initial states uniform on (-2, 2) ([chains,
+  dim_stride], zero past each problem's dim), keyed by (seed, problem key,
+  chain, element) with hipmc_draws kind 4."""
+  x = torch.zeros((batch.n_chains, batch.dim_stride), dtype=batch.dtype,
+                  device=batch.device)
+  sd = np.ascontiguousarray(np.asarray(seed, np.uint32).reshape(2))
+  for i in range(batch.n_problems):
+    a, b = int(batch.chain_offsets[i]), int(batch.chain_offsets[i + 1])
+    d = batch.dims[i]
+    out = torch.empty((b - a, d), dtype=batch.dtype, device=batch.device)
+    _lib.check(_lib.lib().hipmc_draws(
+        _lib.dtype_code(batch.dtype), 4, sd.ctypes.data,
+        int(batch.keys[i]), b - a, d, 0, 0, _lib.ptr(out), None,
+        _lib.stream_handle(batch.device)))
+    x[a:b, :d] = out
+  return x
+
+
+def _windowed_impl(kind, n_draws, joint_dist, *, n_chains,
+                   num_adaptation_steps, current_state, init_step_size,
+                   dual_averaging_kwargs, max_tree_depth, max_energy_diff,
+                   unrolled_leapfrog_steps, num_leapfrog_steps, trace_fn,
+                   default_trace, return_final_kernel_results,
+                   discard_tuning, chain_axis_names, seed, pins):
+  if chain_axis_names is not None:
+    raise NotImplementedError('chains are not sharded across devices here')
+  da = dict(dual_averaging_kwargs)
+  da.pop('num_adaptation_steps', None)
+  tap = da.pop('target_accept_prob')
+  reduce_fn = da.pop('reduce_fn', None)
+  if reduce_fn not in (None, pmath.reduce_log_harmonic_mean_exp):
+    raise NotImplementedError('windowed adaptation reduces accept '
+                              'probabilities with the harmonic mean')
+  da.pop('experimental_reduce_chain_axis_names', None)
+  if da:
+    raise NotImplementedError(f'dual_averaging_kwargs {sorted(da)} are not '
+                              'built (the class defaults are used)')
+  if trace_fn is not None and trace_fn is not default_trace:
+    raise NotImplementedError(
+        'only the default trace function (or None) is traced on the device')
+  chain_shape = (n_chains,) if isinstance(n_chains, int) else tuple(n_chains)
+  c = int(np.prod(chain_shape))
+  a_steps = int(num_adaptation_steps)
+  dtype = _pin_dtype(pins)
+  dev = torch.device('cuda', torch.cuda.current_device())
+  ct = dists.compile_target(joint_dist, dtype=dtype, device=dev, **pins)
+  spec = ct.target
+  batch = batched.PosteriorBatch([spec], c, bijectors=[ct.bijectors],
+                                 keys=[0])
+  base = samplers.sanitize_seed(seed)
+  setup_seed, sample_seed = samplers.split_seed(base, 2)
+  stream = torch.cuda.current_stream(dev)
+  if current_state is None:
+    x0 = init_uniform(batch, setup_seed)
+  else:
+    parts = (list(current_state.values()) if isinstance(current_state, dict)
+             else list(current_state))
+    flat = spec.pack_state([torch.as_tensor(p, dtype=dtype, device=dev)
+                            .reshape((c,) + tuple(torch.as_tensor(p).shape[
+                                len(chain_shape):])) for p in parts])
+    x0 = batch.bijector(flat.contiguous(), 1)
+  if init_step_size is None:
+    step = 0.5 * float(spec.dim) ** -0.25   # `_get_step_size`, `:560-575`
+  else:
+    step = float(np.asarray(init_step_size))
+  n_res = n_draws if discard_tuning else n_draws + a_steps
+  run = batched._Run(
+      batch, stream, n_res, num_burnin_steps=a_steps if discard_tuning else 0,
+      num_adaptation_steps=a_steps, step_size=step, target_accept_prob=tap,
+      max_tree_depth=max_tree_depth, max_energy_diff=max_energy_diff,
+      kernel=kind, num_leapfrog_steps=num_leapfrog_steps, seed=sample_seed,
+      trace=True, adaptation='windowed', init_unconstrained=x0,
+      history=trace_fn is not None,
+      unrolled_leapfrog_steps=unrolled_leapfrog_steps)
+  while not run.done():
+    run.advance()
+  res = run.finish()
+  flat = res.draws[0].reshape((n_res * c, spec.dim))
+  parts = spec.unpack_state(flat, (n_res,) + chain_shape)
+  states = _structure(joint_dist, ct.names, parts)
+  if trace_fn is None and not return_final_kernel_results:
+    return states
+  trace = _trace(kind, run, spec, ct.names, joint_dist, chain_shape, n_res,
+                 a_steps, discard_tuning) if trace_fn is not None else ()
+  if return_final_kernel_results:
+    final = WindowedAdaptationResults(
+        step_size=run.step[0].clone(), momentum_precision=run.prec[0].clone(),
+        running_variance_count=run.rv_n[0].clone(),
+        running_variance_mean=run.rv_mean[0].clone(),
+        running_variance_ssr=run.rv_ssr[0].clone(), step=run.t)
+    return sample_lib.CheckpointableStatesAndTrace(states, trace, final)
+  return sample_lib.StatesAndTrace(states, trace)
+
+
+def _structure(joint_dist, names, parts):
+  if isinstance(joint_dist, dists.JointDistributionNamed):
+    return dict(zip(names, parts))
+  return list(parts)
+
+
+def _trace(kind, run, spec, names, joint_dist, chain_shape, n_res, a_steps,
+           discard_tuning):
+  """The default trace fields for the recorded rows."""
+  t0 = a_steps if discard_tuning else 0
+  rows = slice(t0, t0 + n_res)
+  c = int(np.prod(chain_shape))
+  lar = run.lar[rows].reshape((n_res,) + chain_shape)
+  step_rows = torch.arange(t0, t0 + n_res, device=run.lar.device)
+  # variance_scaling: 1 / precision of the momentum in the results after
+  # each step (the mass matrix is pushed into them at the end of each slow
+  # window)
+  change_at = torch.as_tensor([s for s, _ in run.prec_changes],
+                              device=run.lar.device)
+  which = torch.searchsorted(change_at, step_rows + 1, right=True) - 1
+  precs = torch.stack([p[0, :spec.dim] for _, p in run.prec_changes])
+  vs = (1.0 / precs[which])[:, None, :].expand(n_res, c, spec.dim)
+  vs_parts = spec.unpack_state(vs.reshape(n_res * c, spec.dim),
+                               (n_res,) + chain_shape)
+  out = {
+      'step_size': run.step_hist[rows, 0],
+      'tune': (step_rows + 1) <= a_steps,
+      'target_log_prob': run.tlp_hist[rows].reshape((n_res,) + chain_shape),
+  }
+  if kind == 'nuts':
+    out['diverging'] = run.div[rows].bool().reshape((n_res,) + chain_shape)
+  else:
+    out['diverging'] = lar.abs() > 500.0
+    out['log_acceptance_correction'] = run.corr[rows].reshape(
+        (n_res,) + chain_shape)
+  out['accept_ratio'] = torch.clamp(torch.exp(lar), max=1.0)
+  out['variance_scaling'] = _structure(joint_dist, names, vs_parts)
+  if kind == 'nuts':
+    out['n_steps'] = run.lf[rows].reshape((n_res,) + chain_shape)
+  out['is_accepted'] = run.acc[rows].bool().reshape((n_res,) + chain_shape)
+  return out

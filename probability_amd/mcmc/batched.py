@@ -24,7 +24,8 @@ from probability_amd import bijectors as bij_lib
 from probability_amd import random as samplers
 from probability_amd import targets as targets_lib
 
-__all__ = ['PosteriorBatch', 'sample_posteriors', 'sample_ragged', 'BatchResults']
+__all__ = ['PosteriorBatch', 'sample_posteriors', 'sample_ragged',
+           'BatchResults', 'WindowState', 'window_sizes']
 
 BatchResults = collections.namedtuple('BatchResults', [
     'draws', 'step_size', 'log_accept_ratio', 'leapfrogs_taken',
@@ -262,6 +263,41 @@ def _xcd_order(idx, cost, n_xcd=8):
   return [i for b in bins for i in b]
 
 
+def window_sizes(num_adaptation_steps):
+  """First fast window, first slow window and last fast window of windowed
+  adaptation (`_get_window_sizes`, experimental/mcmc/windowed_sampling.py:
+  367-395): slow windows of s, 2s, 4s, 8s steps with s = A // 20 between a
+  3s-step first window and the rest."""
+  slow = num_adaptation_steps // 20
+  first = 3 * slow
+  return first, slow, num_adaptation_steps - 15 * slow - first
+
+
+class WindowState:
+  """Host bookkeeping of WindowedAdaptation(DiagonalMassMatrixAdaptation(
+  DualAveraging(...))): the dual averaging's own step count and length, and
+  the mass-matrix estimation step and length.  Bootstrapped as the reference
+  builds it: both run for the whole adaptation (`windowed_sampling.py:
+  323-327, 509-520`) until the first window update."""
+
+  def __init__(self, num_adaptation_steps):
+    self.a = int(num_adaptation_steps)
+    self.first, self.slow, self.last = window_sizes(self.a)
+    self.da_step, self.da_num = 0, self.a
+    self.dm_step, self.num_est = 0, self.a
+
+  def option(self, step):
+    """The update applied before inner step `step`
+    (`windowed_sampling.py:471-494`): 1 first fast window, 2 first slow
+    window, 3 a later slow window, 4 the last window; coinciding starts add
+    up and the sum is clipped to 4 (`mcmc_util.choose_from`)."""
+    f, s = self.first, self.slow
+    opt = int(step == 0) + 2 * int(step == f)
+    opt += 3 * int(any(step == f + s * (2 ** i - 1) for i in (1, 2, 3)))
+    opt += 4 * int(step == f + 15 * s)
+    return min(opt, 4)
+
+
 class _Run:
   """One batch's sampling loop, advanced one launch at a time on `stream`."""
 
@@ -269,9 +305,16 @@ class _Run:
                num_adaptation_steps=0, init_state=None, step_size=0.1,
                target_accept_prob=0.75, max_tree_depth=10,
                max_energy_diff=1000.0, kernel='nuts', num_leapfrog_steps=10,
-               seed=None, trace=True, stats=None, max_iters_per_launch=4096):
+               seed=None, trace=True, stats=None, max_iters_per_launch=4096,
+               adaptation='dual_averaging', init_unconstrained=None,
+               history=False, unrolled_leapfrog_steps=1):
     if kernel not in ('nuts', 'hmc'):
       raise ValueError("kernel must be 'nuts' or 'hmc'")
+    if adaptation not in ('dual_averaging', 'windowed'):
+      raise ValueError("adaptation must be 'dual_averaging' or 'windowed'")
+    self.windowed = adaptation == 'windowed'
+    self.history = history
+    self.unrolled = int(unrolled_leapfrog_steps)
     _lib.require_cuda()
     self.batch, self.stream = batch, stream
     self.kernel = kernel
@@ -284,7 +327,10 @@ class _Run:
     dev, dt = batch.device, batch.dtype
     c, d, p = batch.n_chains, batch.dim_stride, batch.n_problems
     with torch.cuda.stream(stream):
-      if init_state is None:
+      if init_unconstrained is not None:
+        self.x = torch.as_tensor(init_unconstrained, dtype=dt,
+                                 device=dev).contiguous().clone()
+      elif init_state is None:
         self.x = torch.zeros((c, d), dtype=dt, device=dev)
       else:
         self.x = batch.bijector(torch.as_tensor(init_state, dtype=dt,
@@ -307,6 +353,24 @@ class _Run:
       self.div = torch.zeros((rows, c), dtype=torch.uint8, device=dev)
       self.tr = (torch.empty((num_results, c, d), dtype=dt, device=dev)
                  if trace else None)
+      # windowed adaptation: diagonal momentum precision per problem, the
+      # running variance it is estimated from, and the window bookkeeping
+      self.prec = None
+      if self.windowed:
+        self.prec = torch.ones((p, d), dtype=dt, device=dev)
+        self.rv_n = torch.zeros(p, dtype=torch.float64, device=dev)
+        self.rv_mean = torch.zeros((p, d), dtype=torch.float64, device=dev)
+        self.rv_ssr = torch.zeros((p, d), dtype=torch.float64, device=dev)
+        self.win = WindowState(num_adaptation_steps)
+        self.prec_changes = [(0, self.prec.clone())]
+      # per-step step size and target log-prob (trace fields)
+      if history:
+        self.step_hist = torch.empty((max(self.n_total, 1), p), dtype=dt,
+                                     device=dev)
+        self.tlp_hist = torch.empty((max(self.n_total, 1), c), dtype=dt,
+                                    device=dev)
+        self.corr = torch.empty((max(self.n_total, 1), c), dtype=dt,
+                                device=dev)
     self.base_seed = samplers.sanitize_seed(seed, salt='mcmc.sample_chain')
     self.fn = (_lib.lib().hipmc_nuts_run if kernel == 'nuts'
                else _lib.lib().hipmc_hmc_run)
@@ -318,14 +382,80 @@ class _Run:
     return self.t >= self.n_total
 
   def advance(self):
-    """Enqueues the next launch (and its dual-averaging update)."""
+    """Enqueues the next launch (and its step-size / mass-matrix updates)."""
     with torch.cuda.stream(self.stream):
       t = self.t
-      n = 1 if t < self.num_adapt else min(self.n_total - t, self.max_iters)
+      adapting = t < self.num_adapt
+      n = 1 if adapting else min(self.n_total - t, self.max_iters)
+      if self.windowed and adapting:
+        self._window_pre(t)
+      if self.history:
+        self.step_hist[t:t + n] = self.step
       self._launch(t, n)
-      if self.num_adapt > 0:
+      if self.windowed:
+        if adapting:
+          self._window_post(t)
+      elif self.num_adapt > 0:
         self._da(t, n)
       self.t = t + n
+
+  # ---- windowed adaptation (experimental/mcmc/windowed_sampling.py:415-500,
+  # diagonal_mass_matrix_adaptation.py:186-244)
+  def _da_reset(self):
+    """DualAveraging._bootstrap_from_inner_results at the current step size
+    (dual_averaging_step_size_adaptation.py:543-610)."""
+    self.err.zero_()
+    self.avg.zero_()
+    self.lst = np.log(10.0) + torch.log(self.step.to(torch.float64))
+    self.win.da_step = 0
+
+  def _window_pre(self, t):
+    w = self.win
+    opt = w.option(t)
+    if opt == 1:
+      w.da_num = w.first + w.slow
+      w.num_est = -1
+    elif opt == 2:
+      w.dm_step = 0
+      w.num_est = w.slow
+    elif opt == 3:
+      cur = t - w.first + w.slow
+      self.rv_n.zero_()
+      self.rv_mean.zero_()
+      self.rv_ssr.zero_()
+      w.dm_step = 0
+      w.num_est = cur
+      self._da_reset()
+      w.da_num = cur
+    elif opt == 4:
+      self._da_reset()
+      w.da_num = w.last
+
+  def _window_post(self, t):
+    b, w = self.batch, self.win
+    _lib.check(_lib.lib().hipmc_da_update(
+        _lib.dtype_code(b.dtype), b.n_problems, None, None,
+        _lib.ptr(b.group_start), _lib.ptr(self.lar[t]), 1, b.n_chains,
+        _lib.ptr(self.err), _lib.ptr(self.avg), _lib.ptr(self.lst),
+        _lib.ptr(self.tgt), w.da_step, w.da_num, 0.05, 10.0, 0.75,
+        _lib.REDUCE_HARMONIC, _lib.ptr(self.step), _lib.ptr(self.step64),
+        _lib.stream_handle(b.device)))
+    w.da_step += 1
+    w.dm_step += 1
+    if w.dm_step <= w.num_est:
+      _lib.check(_lib.lib().hipmc_variance_update(
+          _lib.dtype_code(b.dtype), b.n_problems, _lib.ptr(b.group_start),
+          None, _lib.ptr(self.x), b.dim_stride, b.dim_stride,
+          _lib.ptr(self.rv_n), _lib.ptr(self.rv_mean), _lib.ptr(self.rv_ssr),
+          _lib.stream_handle(b.device)))
+    if w.dm_step == w.num_est:
+      # the momentum precision becomes the population variance
+      self.prec.copy_((self.rv_ssr / self.rv_n[:, None]).to(self.prec.dtype))
+      # padding elements past a problem's dim are never read; keep them 1
+      for i, dd in enumerate(b.dims):
+        if dd < b.dim_stride:
+          self.prec[i, dd:] = 1
+      self.prec_changes.append((t + 1, self.prec.clone()))
 
   def _launch(self, t0, n):
     b, dt = self.batch, self.batch.dtype
@@ -340,7 +470,7 @@ class _Run:
     a.seed[0], a.seed[1] = int(self.base_seed[0]), int(self.base_seed[1])
     a.num_leapfrog_steps = int(self.num_leapfrog_steps)
     a.max_tree_depth = int(self.max_tree_depth)
-    a.unrolled_leapfrog_steps = 1
+    a.unrolled_leapfrog_steps = self.unrolled
     a.target_kind = b.kind
     a.waves_per_chain = b.waves_per_chain
     a.max_energy_diff = float(self.max_energy_diff)
@@ -350,6 +480,12 @@ class _Run:
     a.chain_local = b.chain_local.data_ptr()
     a.step_size = self.step.data_ptr()
     a.chain_step = b.chain_problem.data_ptr()
+    if self.prec is not None:
+      a.momentum_precision = self.prec.data_ptr()
+    if self.history:
+      a.out_target_log_prob = self.tlp_hist.data_ptr()
+      if self.kernel == 'hmc':
+        a.out_energy = self.corr.data_ptr()
     a.state, a.tlp = self.x.data_ptr(), self.tlp.data_ptr()
     a.grad = self.grad.data_ptr()
     a.out_row0 = t0
@@ -379,7 +515,7 @@ class _Run:
         _lib.ptr(b.group_start), _lib.ptr(self.lar[t0]), n, b.n_chains,
         _lib.ptr(self.err), _lib.ptr(self.avg), _lib.ptr(self.lst),
         _lib.ptr(self.tgt), t0, self.num_adapt, 0.05, 10.0, 0.75,
-        _lib.ptr(self.step), _lib.ptr(self.step64),
+        _lib.REDUCE_MEAN, _lib.ptr(self.step), _lib.ptr(self.step64),
         _lib.stream_handle(b.device)))
 
   def finish(self, consumer=None):

@@ -14,6 +14,7 @@ import numpy as np
 import torch
 
 from probability_amd import _lib
+from probability_amd import math as pmath
 from probability_amd.mcmc import kernel as kernel_base
 from probability_amd.mcmc import simple_step_size_adaptation as ssa
 from probability_amd.mcmc.internal import util as mcmc_util
@@ -34,6 +35,24 @@ class DualAveragingStepSizeAdaptationResults(
   error_sum has the shape of its step-size part (one copy per element).
   """
   __slots__ = ()
+
+
+def reduce_mode(reduce_fn):
+  """The device reduction for a `reduce_fn` argument: None or
+  `reduce_logmeanexp` (the reference's default) or
+  `reduce_log_harmonic_mean_exp` (what windowed adaptation passes,
+  `experimental/mcmc/windowed_sampling.py:869-873`).  Other callables are
+  not built and raise."""
+  if reduce_fn is None or reduce_fn is pmath.reduce_logmeanexp:
+    return _lib.REDUCE_MEAN
+  fn = getattr(reduce_fn, 'func', reduce_fn)  # functools.partial
+  if fn is pmath.reduce_logmeanexp:
+    return _lib.REDUCE_MEAN
+  if fn is pmath.reduce_log_harmonic_mean_exp:
+    return _lib.REDUCE_HARMONIC
+  raise NotImplementedError(
+      'reduce_fn: only reduce_logmeanexp and reduce_log_harmonic_mean_exp '
+      'are built')
 
 
 def chain_groups(lap_shape, step_shape, state_rank):
@@ -97,8 +116,7 @@ class DualAveragingStepSizeAdaptation(kernel_base.TransitionKernel):
                log_accept_prob_getter_fn=ssa.hmc_like_log_accept_prob_getter_fn,
                reduce_fn=None, experimental_reduce_chain_axis_names=None,
                validate_args=False, name=None):
-    if reduce_fn is not None:
-      raise NotImplementedError('only the default reduce_logmeanexp is built')
+    reduce_mode(reduce_fn)  # raises for a reduction that is not built
     if experimental_reduce_chain_axis_names is not None:
       raise NotImplementedError('cross-device chain reduction is not built')
     if validate_args:
@@ -209,7 +227,8 @@ class DualAveragingStepSizeAdaptation(kernel_base.TransitionKernel):
         _lib.ptr(avg), _lib.ptr(lst), _lib.ptr(tgt), step_no,
         int(pkr.num_adaptation_steps), float(pkr.exploration_shrinkage),
         float(pkr.step_count_smoothing), float(pkr.decay_rate),
-        _lib.ptr(new_step), _lib.ptr(step64), _lib.stream_handle(device)))
+        reduce_mode(self.parameters['reduce_fn']), _lib.ptr(new_step),
+        _lib.ptr(step64), _lib.stream_handle(device)))
     return new_step.reshape(shape), err.reshape(shape), avg.reshape(shape)
 
   def bootstrap_results(self, init_state):

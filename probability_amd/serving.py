@@ -338,8 +338,8 @@ class SamplerServer:
         _lib.dtype_code(self.dtype), n_el, _lib.ptr(sg), _lib.ptr(si),
         _lib.ptr(gstart), _lib.ptr(lar), 1, n, _lib.ptr(pool.err),
         _lib.ptr(pool.avg), _lib.ptr(pool.lst), _lib.ptr(pool.tgt), step_no,
-        req.num_adaptation_steps, 0.05, 10.0, 0.75, _lib.ptr(pool.step),
-        _lib.ptr(pool.step64), stream.cuda_stream))
+        req.num_adaptation_steps, 0.05, 10.0, 0.75, _lib.REDUCE_MEAN,
+        _lib.ptr(pool.step), _lib.ptr(pool.step64), stream.cuda_stream))
 
   def tick(self):
     """Admits what fits, then for every lane whose launch has finished
