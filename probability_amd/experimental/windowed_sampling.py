@@ -129,19 +129,7 @@ def init_uniform(batch, seed):
   """Unconstrained initial states uniform on (-2, 2) ([chains,
   dim_stride], zero past each problem's dim), keyed by (seed, problem key,
   chain, element) with hipmc_draws kind 4."""
-  x = torch.zeros((batch.n_chains, batch.dim_stride), dtype=batch.dtype,
-                  device=batch.device)
-  sd = np.ascontiguousarray(np.asarray(seed, np.uint32).reshape(2))
-  for i in range(batch.n_problems):
-    a, b = int(batch.chain_offsets[i]), int(batch.chain_offsets[i + 1])
-    d = batch.dims[i]
-    out = torch.empty((b - a, d), dtype=batch.dtype, device=batch.device)
-    _lib.check(_lib.lib().hipmc_draws(
-        _lib.dtype_code(batch.dtype), 4, sd.ctypes.data,
-        int(batch.keys[i]), b - a, d, 0, 0, _lib.ptr(out), None,
-        _lib.stream_handle(batch.device)))
-    x[a:b, :d] = out
-  return x
+  return batched._init_uniform(batch, seed)
 
 
 def _windowed_impl(kind, n_draws, joint_dist, *, n_chains,

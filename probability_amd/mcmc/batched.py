@@ -184,6 +184,12 @@ def sample_ragged(specs, num_chains, num_results, bijectors=None, keys=None,
     raise ValueError('need at least one problem')
   specs = [targets_lib.as_target(s) for s in specs]
   p = len(specs)
+  # windowed adaptation starts every chain uniform on (-2, 2) from the setup
+  # half of the seed, as windowed_adaptive_nuts does
+  windowed_init = kwargs.get('adaptation') == 'windowed'
+  if windowed_init:
+    setup_seed, kwargs['seed'] = samplers.split_seed(
+        samplers.sanitize_seed(kwargs.get('seed')), 2)
   chains = ([int(num_chains)] * p if np.isscalar(num_chains)
             else [int(c) for c in num_chains])
   bijs = bijectors or [None] * p
@@ -209,6 +215,11 @@ def sample_ragged(specs, num_chains, num_results, bijectors=None, keys=None,
     runs.append((idx, sub, _Run(batch, stream, num_results, stats=sub,
                                 **kwargs)))
   total = sum(r.n_total for _, _, r in runs)
+  if windowed_init:
+    for _, _, r in runs:
+      with torch.cuda.stream(r.stream):
+        r.x.copy_(_init_uniform(r.batch, setup_seed))
+        r.tlp, r.grad = r.batch.eval(r.x)
   while not all(r.done() for _, _, r in runs):
     for _, _, r in runs:
       if not r.done():
@@ -245,6 +256,24 @@ def sample_ragged(specs, num_chains, num_results, bijectors=None, keys=None,
   if not kwargs.get('trace', True):
     out['draws'] = None
   return BatchResults(**out)
+
+
+def _init_uniform(batch, seed):
+  """Unconstrained states uniform on (-2, 2), keyed per (problem, chain)
+  (hipmc_draws kind 4); zero past each problem's dim."""
+  x = torch.zeros((batch.n_chains, batch.dim_stride), dtype=batch.dtype,
+                  device=batch.device)
+  sd = np.ascontiguousarray(np.asarray(seed, np.uint32).reshape(2))
+  for i in range(batch.n_problems):
+    a, b = int(batch.chain_offsets[i]), int(batch.chain_offsets[i + 1])
+    d = batch.dims[i]
+    out = torch.empty((b - a, d), dtype=batch.dtype, device=batch.device)
+    _lib.check(_lib.lib().hipmc_draws(
+        _lib.dtype_code(batch.dtype), 4, sd.ctypes.data, int(batch.keys[i]),
+        b - a, d, 0, 0, _lib.ptr(out), None,
+        _lib.stream_handle(batch.device)))
+    x[a:b, :d] = out
+  return x
 
 
 def _xcd_order(idx, cost, n_xcd=8):
