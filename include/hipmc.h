@@ -294,6 +294,23 @@ typedef struct hipmc_glm_hmc_args {
 
 int hipmc_glm_hmc_run(const hipmc_glm_hmc_args* args, uint64_t stream);
 
+/*
+ * hipmc_glm_hmc_run without caller workspaces, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  It has no counterpart in the
+ * reference.  The ws_* fields of `args` are ignored: the workspaces are
+ * carved from a static device array of HIPMC_NOALLOC_SCRATCH_BYTES that
+ * each GPU the library runs on holds until the process ends; the entry point
+ * calls no allocator.  Needs 256-byte-rounded sizes of n_split*n_chains*256
+ * float32 + n_split*n_chains float32 + 2*n_chains*256 float32 +
+ * 2*n_chains*256 bf16 + n_chains float64 to fit, else HIPMC_E_UNSUPPORTED.
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (other streams or threads) race.  Outputs equal hipmc_glm_hmc_run's.
+ */
+#define HIPMC_NOALLOC_SCRATCH_BYTES (64u << 20)
+int hipmc_glm_hmc_run_noalloc(const hipmc_glm_hmc_args* args,
+                              uint64_t stream);
+
 const char* hipmc_last_error(void);
 int hipmc_abi_version(void);
 

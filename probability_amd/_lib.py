@@ -30,7 +30,7 @@ EXPORTED_SYMBOLS = (
     'hipmc_leapfrog_run', 'hipmc_nuts_run', 'hipmc_target_eval',
     'hipmc_da_update', 'hipmc_simple_step_update', 'hipmc_mh_accept',
     'hipmc_bijector', 'hipmc_draws', 'hipmc_moments_update',
-    'hipmc_variance_update')
+    'hipmc_variance_update', 'hipmc_glm_hmc_run_noalloc')
 
 _vp = ctypes.c_void_p
 _i32 = ctypes.c_int32
@@ -123,6 +123,8 @@ def lib():
       for name in ('hipmc_hmc_run', 'hipmc_leapfrog_run', 'hipmc_nuts_run'):
         getattr(l, name).argtypes = [ctypes.POINTER(RunArgs), _u64]
       l.hipmc_glm_hmc_run.argtypes = [ctypes.POINTER(GlmHmcArgs), _u64]
+      l.hipmc_glm_hmc_run_noalloc.argtypes = [ctypes.POINTER(GlmHmcArgs),
+                                              _u64]
       l.hipmc_target_eval.argtypes = [_i32, _i32, _vp, _vp, _i32, _i32, _i32,
                                       _vp, _vp, _vp, _u64]
       l.hipmc_da_update.argtypes = [_i32, _i32, _vp, _vp, _vp, _vp, _i32,

@@ -582,6 +582,10 @@ int hipmc_glm_hmc_run(const hipmc_glm_hmc_args* a, uint64_t stream) {
   return launch_glm_hmc(a, as_stream(stream));
 }
 
+int hipmc_glm_hmc_run_noalloc(const hipmc_glm_hmc_args* a, uint64_t stream) {
+  return launch_glm_hmc_noalloc(a, as_stream(stream));
+}
+
 int hipmc_nuts_run(const hipmc_run_args* a, uint64_t stream) {
   int rc = check_run(a, true);
   if (rc) return rc;

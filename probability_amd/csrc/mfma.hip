@@ -4,14 +4,49 @@
 // reduction, prior, leapfrog / MH arithmetic).
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+#include <string>
+
 #include "../../include/hipmc.h"
 #include "glm_mfma.h"
 #include "runtime.h"
+
+// Workspace of hipmc_glm_hmc_run_noalloc: one array per GPU the library runs
+// on (each device's copy of the code object holds its own), kept until the
+// process ends.  Host code must not use this name as an address: the device
+// address comes from hipGetSymbolAddress (noalloc_scratch below).
+static __device__ __align__(256) unsigned char scratch[HIPMC_NOALLOC_SCRATCH_BYTES];
 
 namespace hipmc {
 
 namespace {
 int blocks_for(size_t n, int bs) { return (int)((n + bs - 1) / bs); }
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices] = {};
+
+// Device address of the current device's scratch array, looked up once per
+// device (with that device current) on its first call.
+int noalloc_scratch(unsigned char** out) {
+  int dev = 0;
+  int rc = hip_check(hipGetDevice(&dev), "hipGetDevice");
+  if (rc != HIPMC_OK) return rc;
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(HIPMC_E_UNSUPPORTED, "glm_hmc_noalloc: device index too large");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    rc = hip_check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)),
+                   "hipGetSymbolAddress(scratch)");
+    if (rc != HIPMC_OK) return rc;
+    g_scratch[dev] = p;
+  }
+  *out = (unsigned char*)g_scratch[dev];
+  return HIPMC_OK;
+}
+
+size_t round256(size_t n) { return (n + 255) & ~(size_t)255; }
 }  // namespace
 
 int launch_glm_hmc(const hipmc_glm_hmc_args* a, hipStream_t s) {
@@ -67,6 +102,45 @@ int launch_glm_hmc(const hipmc_glm_hmc_args* a, hipStream_t s) {
     }
   }
   return hip_check(hipGetLastError(), "glm_hmc launch");
+}
+
+int launch_glm_hmc_noalloc(const hipmc_glm_hmc_args* a, hipStream_t s) {
+  if (!a) return fail(HIPMC_E_INVALID, "glm_hmc_noalloc: args is NULL");
+  if (a->n_chains <= 0 || a->n_split < 1 || a->dim != kMD)
+    return launch_glm_hmc(a, s);  // the shared argument checks report it
+  // workspace layout (each piece 256-byte aligned): ws_g [n_split, C, D]
+  // f32, ws_ll [n_split, C] f32, ws_q and ws_p [C, D] f32, ws_b [2, C, D]
+  // bf16, ws_ke0 [C] f64
+  const size_t c = (size_t)a->n_chains, d = (size_t)kMD;
+  const size_t sizes[6] = {(size_t)a->n_split * c * d * 4,
+                           (size_t)a->n_split * c * 4, c * d * 4, c * d * 4,
+                           2 * c * d * 2, c * 8};
+  size_t need = 0;
+  for (size_t v : sizes) need += round256(v);
+  if (need > (size_t)HIPMC_NOALLOC_SCRATCH_BYTES)
+    return fail(HIPMC_E_UNSUPPORTED,
+                "glm_hmc_noalloc: n_chains x n_split needs " +
+                    std::to_string(need) + " bytes of workspace, more than "
+                    "the library's " +
+                    std::to_string((size_t)HIPMC_NOALLOC_SCRATCH_BYTES) +
+                    "-byte static scratch");
+  unsigned char* base = nullptr;
+  int rc = noalloc_scratch(&base);
+  if (rc != HIPMC_OK) return rc;
+  hipmc_glm_hmc_args b = *a;
+  size_t off = 0;
+  void* ws[6];
+  for (int i = 0; i < 6; ++i) {
+    ws[i] = base + off;
+    off += round256(sizes[i]);
+  }
+  b.ws_g = (float*)ws[0];
+  b.ws_ll = (float*)ws[1];
+  b.ws_q = (float*)ws[2];
+  b.ws_p = (float*)ws[3];
+  b.ws_b = ws[4];
+  b.ws_ke0 = (double*)ws[5];
+  return launch_glm_hmc(&b, s);
 }
 
 }  // namespace hipmc

@@ -43,7 +43,9 @@ int launch_eval_k(int k, const hipmc_problem* p, const int32_t* cp, int n,
                   int stride, const void* x, void* tlp, void* grad,
                   hipStream_t s);
 
-// The BF16 MFMA logistic-regression HMC path (mfma.hip).
+// The BF16 MFMA logistic-regression HMC path (mfma.hip), with caller
+// workspaces or with the library's static scratch.
 int launch_glm_hmc(const hipmc_glm_hmc_args* a, hipStream_t s);
+int launch_glm_hmc_noalloc(const hipmc_glm_hmc_args* a, hipStream_t s);
 
 }  // namespace hipmc
