@@ -394,15 +394,17 @@ __global__ void moments_cross_kernel(const T* __restrict__ x, int64_t n_batch,
                                      int64_t ev, int64_t chunk_n,
                                      double n_old,
                                      const double* __restrict__ mean_in,
-                                     const double* ssr_in, double* ssr_out) {
+                                     const double* ssr_in, double* ssr_out,
+                                     int paired) {
   const double cn = (double)chunk_n, n_new = n_old + cn;
   const double adj = n_old * cn / n_new;
   const int64_t n_el = n_batch * ev;
   const uint32_t e32 = (uint32_t)ev, ev2 = e32 * e32;
   const uint32_t stride = gridDim.x * blockDim.x;
-  if ((e32 & 1u) == 0) {
-    // even ev: each thread takes two neighbours (j, j + 1) of one row, with
-    // 16-byte accesses to the sums and the mean (j is even, so j + 1 < ev)
+  if (paired) {
+    // even ev and 16-byte aligned sums: each thread takes two neighbours
+    // (j, j + 1) of one row, with 16-byte accesses to the sums and the mean
+    // (j is even, so j + 1 < ev)
     const uint32_t n_pairs = ev2 >> 1;
     for (int64_t b = blockIdx.y; b < n_batch; b += gridDim.y) {
       const size_t base = (size_t)b * ev2;
@@ -517,9 +519,16 @@ static int moments_launch(int32_t mode, int64_t n_batch, int64_t ev,
     const int64_t ev2 = ev * ev;
     const unsigned gx = (unsigned)((ev2 + 255) / 256 < 64 ? (ev2 + 255) / 256 : 64);
     const unsigned gy = (unsigned)(n_batch < 65535 ? n_batch : 65535);
+    // the paired path reads mean and sums 16 bytes at a time: it needs an
+    // even event size and 16-byte aligned arrays (else the scalar path)
+    const auto a16 = [](const void* p) {
+      return ((uintptr_t)p & 15u) == 0;
+    };
+    const int paired = (ev % 2 == 0) && a16(mean_in) && a16(ssr_in) &&
+                       a16(ssr_out);
     hipLaunchKernelGGL(moments_cross_kernel<T>, dim3(gx, gy), dim3(256), 0, s,
                        x, n_batch, ev, chunk_n, n_old, mean_in, ssr_in,
-                       ssr_out);
+                       ssr_out, paired);
     const int rc = hip_check(hipGetLastError(), "moments_cross_kernel launch");
     if (rc) return rc;
   }

@@ -71,8 +71,18 @@ def _moments_update(mode, mean, ssr, num_samples, sample, event_ndims,
   if mode == _MODE_COV:  # [n_batch, ev] samples, [n_batch, ev, ev] sums
     ev = math.prod(shape[len(shape) - event_ndims:])
     n_batch = math.prod(shape[:len(shape) - event_ndims])
+    ssr_shape = shape + shape[len(shape) - event_ndims:]
   else:  # element-wise
     ev, n_batch = math.prod(shape), 1
+    ssr_shape = shape
+  # the kernels read float64, contiguous sums on the sample's device
+  dev = sample.device
+  mean = mean.to(device=dev, dtype=torch.float64).contiguous()
+  if ssr is not None:
+    ssr = ssr.to(device=dev, dtype=torch.float64).contiguous()
+    if tuple(ssr.shape) != tuple(ssr_shape):
+      raise ValueError(f'sum of squared residuals has shape '
+                       f'{tuple(ssr.shape)}, expected {tuple(ssr_shape)}')
   if ev == 0 or n_batch == 0:
     return chunk_n, mean.clone(), None if ssr is None else ssr.clone()
   new_mean = torch.empty_like(mean)

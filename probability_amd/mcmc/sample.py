@@ -115,7 +115,10 @@ def sample_chain(num_results, current_state, previous_kernel_results=None,
                   'value) or an explicit callback that traces the values '
                   'you are interested in.')
   base_seed = samplers.sanitize_seed(seed, salt='mcmc.sample_chain')
-  if experimental_fused and previous_kernel_results is None:
+  # with no step to take the step loop returns the bootstrap results as
+  # they are, as the reference's loop of zero iterations does
+  if (experimental_fused and previous_kernel_results is None and
+      num_results + num_burnin_steps > 0):
     plan = _fused_plan(kernel, trace_fn)
     if plan is not None:
       out = _run_fused(plan, num_results, current_state, num_burnin_steps,
@@ -257,8 +260,11 @@ def _run_fused(plan, num_results, current_state, num_burnin_steps, thin,
   trace_state = torch.empty((num_results, c, d), dtype=dt, device=dev)
   step_hist = (torch.empty((rows,) + tuple(step.shape), dtype=dt, device=dev)
                if 'step_size' in want else None)
-  tlp_hist = (torch.empty((rows, c), dtype=dt, device=dev)
-              if 'target_log_prob' in want else None)
+  tlp_hist = None
+  if 'target_log_prob' in want:
+    # every launch writes each iteration's post-step target log-prob
+    tlp_hist = outs['out_target_log_prob'] = torch.empty(
+        (rows, c), dtype=dt, device=dev)
 
   if plan.da is not None:
     n_adapt = int(pkr.num_adaptation_steps)
@@ -311,8 +317,6 @@ def _run_fused(plan, num_results, current_state, num_burnin_steps, thin,
       _launch_fused(plan, spec, table, state, tlp, grad, step, base_seed, n,
                     t, lar_all, outs, trace_state, num_burnin_steps, thin,
                     num_results, fn, stream, events)
-    if tlp_hist is not None and n == 1:
-      tlp_hist[t] = tlp
     if plan.da is not None:
       _lib.check(_lib.lib().hipmc_da_update(
           _lib.dtype_code(dt), gplan.n_steps, _lib.ptr(gplan.step_group),
@@ -368,7 +372,7 @@ def _run_fused(plan, num_results, current_state, num_burnin_steps, thin,
       elif name == 'step_size':
         v = step_hist[rec_idx]
       elif name == 'target_log_prob':
-        raise NotImplementedError('target_log_prob is traced by the step loop')
+        v = tlp_hist[rec_idx]
       if name != 'step_size':
         v = v.reshape((num_results,) + cs + tuple(v.shape[2:]))
       elif step_shape is not None:

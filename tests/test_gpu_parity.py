@@ -466,3 +466,33 @@ def test_sample_chain_nuts_da_ttk_matches_oracle(tdt, ndt, tol):
   np.testing.assert_array_equal(_np(res.trace[1]), lf)
   np.testing.assert_allclose(got, ostates, rtol=2e-4 if ndt == np.float32
                              else 1e-9, atol=2e-4 if ndt == np.float32 else 1e-9)
+
+
+@pytest.mark.parametrize('tdt,ndt,tol', DTYPES)
+def test_fused_trace_of_target_log_prob_equals_step_loop(tdt, ndt, tol):
+  # the kernels write each iteration's target log-prob, so the fused loop
+  # traces it in its multi-iteration launches too
+  spec, da, init, _, _ = _hier_kernel(tdt, ndt)
+  tf = pmcmc.TraceFields('target_log_prob', 'energy')
+  kw = dict(num_burnin_steps=8, num_steps_between_results=2, trace_fn=tf,
+            seed=6)
+  a = pmcmc.sample_chain(10, init, kernel=da, **kw)
+  b = pmcmc.sample_chain(10, init, kernel=da, experimental_fused=False, **kw)
+  assert tuple(a.trace[0].shape) == (10,) + tuple(init[0].shape)
+  np.testing.assert_array_equal(_np(a.trace[0]), _np(b.trace[0]))
+  np.testing.assert_array_equal(_np(a.trace[1]), _np(b.trace[1]))
+
+
+def test_zero_step_sample_chain_returns_bootstrap_results():
+  spec, da, init, _, _ = _hier_kernel(torch.float32, np.float32)
+  out = pmcmc.sample_chain(0, init, kernel=da, num_burnin_steps=0,
+                           trace_fn=None, seed=3,
+                           return_final_kernel_results=True)
+  boot = da.bootstrap_results(init)
+  fk = out.final_kernel_results
+  inner, binner = fk.inner_results.inner_results, boot.inner_results.inner_results
+  np.testing.assert_array_equal(_np(inner.target_log_prob),
+                                _np(binner.target_log_prob))
+  np.testing.assert_array_equal(_np(inner.log_accept_ratio),
+                                _np(binner.log_accept_ratio))
+  np.testing.assert_array_equal(_np(inner.energy), _np(binner.energy))

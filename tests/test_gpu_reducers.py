@@ -221,3 +221,52 @@ def test_sample_discarding_kernel_over_hmc():
   # the same inner steps by hand: 2 burn-in + 1 thinned + 1 kept
   want = emcmc.step_kernel(4, q0, kernel=kern, seed=11)
   assert torch.equal(q, want) and kr.call_counter == 1
+
+
+@pytest.mark.parametrize('tdt,ndt,tol', DTYPES)
+@pytest.mark.parametrize('chunks', [None, [4, 25, 11]])
+def test_running_covariance_even_event_many_batch_members(tdt, ndt, tol,
+                                                          chunks):
+  # shape (5, 6), event_ndims 1: five batch members, an even event size
+  # (the kernel's paired 16-byte path with blockIdx.y over members)
+  x = np.random.default_rng(4).normal(size=(40, 5, 6)).astype(ndt)
+  x[..., 3] -= 0.7 * x[..., 2]
+  rc = estats.RunningCovariance.from_shape((5, 6), tdt, 1)
+  if chunks is None:
+    for r in range(40):
+      rc = rc.update(_device(x[r], tdt))
+  else:
+    at = 0
+    for c in chunks:
+      rc = rc.update(_device(x[at:at + c], tdt), axis=0)
+      at += c
+  m = _oracle(x.astype(np.float64), 1, 2, chunks)
+  np.testing.assert_allclose(rc.covariance().cpu().numpy(),
+                             ored.covariance(m), rtol=tol, atol=tol)
+  np.testing.assert_allclose(rc.mean.cpu().numpy(), m.mean, rtol=tol,
+                             atol=tol)
+
+
+@pytest.mark.parametrize('offset', [0, 1])
+def test_moments_update_in_place_and_unaligned(offset):
+  # through the C ABI: an in-place update (out == in), and sums starting
+  # one float64 past a 16-byte boundary (the scalar cross-product path)
+  from probability_amd import _lib
+  r = np.random.default_rng(6)
+  nb, ev, n = 3, 4, 9
+  x = r.normal(size=(n, nb, ev))
+  dev = 'cuda'
+  mean_buf = torch.zeros(nb * ev + offset, dtype=torch.float64, device=dev)
+  ssr_buf = torch.zeros(nb * ev * ev + offset, dtype=torch.float64,
+                        device=dev)
+  mean, ssr = mean_buf[offset:], ssr_buf[offset:]
+  xt = torch.as_tensor(x, device=dev).contiguous()
+  _lib.check(_lib.lib().hipmc_moments_update(
+      _lib.dtype_code(torch.float64), 2, nb, ev, n, _lib.ptr(xt), 0.0,
+      _lib.ptr(mean), _lib.ptr(ssr), _lib.ptr(mean), _lib.ptr(ssr), 0))
+  m = ored.Moments((nb, ev), 1, 2)
+  m = ored.update(m, x, axis=0)
+  np.testing.assert_allclose(mean.cpu().numpy().reshape(nb, ev), m.mean,
+                             rtol=1e-12, atol=1e-14)
+  np.testing.assert_allclose(ssr.cpu().numpy().reshape(nb, ev, ev), m.ssr,
+                             rtol=1e-12, atol=1e-13)
