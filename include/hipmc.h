@@ -247,6 +247,17 @@ typedef struct hipmc_run_args {
    * preconditioned_nuts.py:804-1010); NUTS then reports energy including
    * the momentum density's normalising terms.  NULL: identity. */
   const void* momentum_precision;
+  /* hipmc_nuts_run on LOGISTIC / LINEAR problems: chains that share each
+   * pass over X (0 or 1: none).  With 2 or 4, blocks of waves_per_chain
+   * waves run that many consecutive chains (fewer where the LDS does not
+   * fit, and none for states above 192 elements or more than one leapfrog
+   * per leaf), and every group of chain_problem must lie within one
+   * problem: each problem's chains are consecutive and their count a
+   * multiple of chains_per_block.  Results equal each chain's own run up to
+   * the order of floating-point sums (the group's partial sums are added in
+   * a fixed order, so a chain's results depend on its group size only). */
+  int32_t chains_per_block;
+  int32_t pad0_;
 } hipmc_run_args;
 
 /*

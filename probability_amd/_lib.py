@@ -83,6 +83,7 @@ class RunArgs(ctypes.Structure):
       ('out_initial_momentum', _vp), ('out_final_momentum', _vp),
       ('streams', _vp), ('chain_row', _vp),
       ('out_target_log_prob', _vp), ('momentum_precision', _vp),
+      ('chains_per_block', _i32), ('pad0_', _i32),
   ]
 
 

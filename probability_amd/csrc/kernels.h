@@ -185,8 +185,8 @@ __device__ inline void store_row(T* row, const T (&v)[K], int dim) {
 // velocity prec * p of the momentum distribution (identity: prec = 1):
 //   p += (0.5*eps)*g; repeat: q += eps*(prec*p); (t, g) = grad(q);
 //   p += eps*g; p -= (0.5*eps)*g.
-template <typename T, int K, int FAM>
-__device__ inline void leapfrog(Target<T, K, FAM>& tg, T eps, int num_steps,
+template <typename T, int K, int FAM, int G>
+__device__ inline void leapfrog(Target<T, K, FAM, G>& tg, T eps, int num_steps,
                                 const T (&m)[K], T (&q)[K], T (&p)[K],
                                 T (&g)[K], double& t) {
   const T half = T(0.5) * eps;
@@ -340,10 +340,10 @@ __host__ __device__ constexpr size_t nuts_lds_elems(int max_depth) {
 // W > 1: the W waves of one block share the chain (GLMs); they run the same
 // control flow on identical values, and their parked tree vectors are one
 // copy guarded by a block barrier before each write.
-template <typename T, int K, int FAM, int BM, bool U1>
+template <typename T, int K, int FAM, int BM, bool U1, int G = 1>
 __device__ __attribute__((always_inline)) inline void nuts_chain(
-    const hipmc_run_args& a, Target<T, K, FAM>& tg, const ChainId& id, int c,
-    int wic, int W, T* memP, T* memC, T* tree, T* leafp,
+    const hipmc_run_args& a, Target<T, K, FAM, G>& tg, const ChainId& id,
+    int c, int wic, int W, T* memP, T* memC, T* tree, T* leafp,
     const hipmc_problem& pb) {
   constexpr bool PARK = parks<FAM>();
   constexpr size_t S = (size_t)K * 64;
@@ -663,6 +663,60 @@ __global__ void __launch_bounds__(FAM == kFamilyGlm ? 512 : 256)
     nuts_chain<T, K, FAM, kBijAny, false>(a, tg, id, c, wic, W, memP, memC,
                                           tree, leafp, pb);
   }
+}
+
+// Chain groups (GLM): G chains of one problem per block, chain j's NUTS run
+// by wave j ("owner"), every gradient a round in which all W waves of the
+// block pass over X once for all G chains (Target<..., G>::group_round).
+// Rounds hold two block barriers; an owner whose chain has finished, and
+// every other wave, keep joining rounds (group_finish) until all owners are
+// done, so every wave reaches the same barriers.  The chains of a block must
+// belong to one problem (the host lays the launch out so).
+// LDS: per chain its U-turn memory and parked tree vectors, per owner its
+// parked leaf vectors, then the group's shared buffers (GroupLayout).
+template <typename T, int K>
+__host__ __device__ constexpr size_t nuts_group_lds_bytes(int max_depth, int g,
+                                                         int w) {
+  return ((size_t)g * (nuts_chain_lds_elems<T, K, kFamilyGlm>(max_depth) +
+                       (size_t)kLeafSlots * K * 64)) * sizeof(T) +
+         GroupLayout<T, K>::bytes(g, w);
+}
+
+template <typename T, int K, int G>
+__global__ void __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(2)))
+    nuts_group_kernel(hipmc_run_args a) {
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  const int wave = threadIdx.x >> 6;
+  const int W = blockDim.x >> 6;
+  const int max_depth = a.max_tree_depth;
+  const int c0 = xcd_block(blockIdx.x, gridDim.x) * G;
+  const size_t cl = nuts_chain_lds_elems<T, K, kFamilyGlm>(max_depth);
+  const size_t ll = (size_t)kLeafSlots * K * 64;
+  T* base = (T*)smem_raw;
+  unsigned char* shared = smem_raw + (size_t)G * (cl + ll) * sizeof(T);
+  const int j = wave < G ? wave : -1;
+  const bool active = j >= 0 && c0 + j < a.n_chains;
+  // the group's problem: that of its first chain (every chain of the block
+  // shares it)
+  const ChainId id0 = chain_of(a, c0);
+  const hipmc_problem& pb = a.problems[id0.prob];
+  Target<T, K, kFamilyGlm, G> tg;
+  tg.load(pb, nullptr);
+  tg.set_group(j, wave, W, shared);
+  if (j >= 0 && lane_id() == 0)
+    ((int*)(shared + GroupLayout<T, K>::done(G)))[j] = active ? 0 : 1;
+  if (active) {
+    T* cb = base + (size_t)j * (cl + ll);
+    T* memP = cb;
+    T* memC = cb + (size_t)max_depth * K * 64;
+    T* tree = cb + (size_t)2 * max_depth * K * 64;
+    T* leafp = cb + cl;
+    const int c = c0 + j;
+    const ChainId id = chain_of(a, c);
+    nuts_chain<T, K, kFamilyGlm, kBijAny, true, G>(a, tg, id, c, 0, 1, memP,
+                                                   memC, tree, leafp, pb);
+  }
+  tg.group_finish();
 }
 
 // ------------------------------------------------------------------ HMC

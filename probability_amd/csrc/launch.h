@@ -26,9 +26,51 @@ inline int waves_for_lds(size_t per_wave_bytes) {
   return w;
 }
 
+// Chain groups of G chains on W waves; HIPMC_E_UNSUPPORTED when the LDS does
+// not fit (the caller then tries a smaller group).
+template <typename T, int K, int G>
+int launch_nuts_group(const hipmc_run_args& a, int W, hipStream_t s) {
+  const size_t bytes = nuts_group_lds_bytes<T, K>(a.max_tree_depth, G, W);
+  if (bytes > (size_t)kMaxLdsSharedChain)
+    return fail(HIPMC_E_UNSUPPORTED, "chain group LDS does not fit");
+  const int rc = allow_lds(&nuts_group_kernel<T, K, G>, bytes);
+  if (rc != HIPMC_OK) return rc;
+  const int blocks = (a.n_chains + G - 1) / G;
+  hipLaunchKernelGGL((nuts_group_kernel<T, K, G>), dim3(blocks), dim3(64 * W),
+                     bytes, s, a);
+  return hip_check(hipGetLastError(), "nuts_group_kernel launch");
+}
+
+// Grouped launch of a GLM NUTS run when a.chains_per_block asks for it
+// (slot counts up to 3, one leapfrog per leaf): the largest group of at most
+// chains_per_block (4 or 2) chains, no more than the waves, whose LDS fits.
+// Returns 1 when nothing was launched (the caller runs the per-chain
+// kernel).
+template <typename T, int K, int FAM>
+int try_launch_nuts_group(const hipmc_run_args& a, hipStream_t s) {
+  if constexpr (FAM != kFamilyGlm || K > 3) {
+    return 1;
+  } else {
+    if (a.chains_per_block < 2 || a.unrolled_leapfrog_steps != 1) return 1;
+    const int W = waves_per_chain<FAM>(a);
+    if (a.chains_per_block >= 4 && W >= 4 &&
+        nuts_group_lds_bytes<T, K>(a.max_tree_depth, 4, W) <=
+            (size_t)kMaxLdsSharedChain)
+      return launch_nuts_group<T, K, 4>(a, W, s);
+    if (W >= 2 && nuts_group_lds_bytes<T, K>(a.max_tree_depth, 2, W) <=
+                      (size_t)kMaxLdsSharedChain)
+      return launch_nuts_group<T, K, 2>(a, W, s);
+    return 1;
+  }
+}
+
 template <typename T, int K, int FAM>
 int launch_nuts(const hipmc_run_args& a0, hipStream_t s) {
   if (a0.n_chains == 0) return HIPMC_OK;
+  {
+    const int rc = try_launch_nuts_group<T, K, FAM>(a0, s);
+    if (rc != 1) return rc;
+  }
   // one chain per block: the chain's LDS (U-turn memory, parked tree
   // vectors), W wave blocks (target scratch, parked leaf vectors) and the
   // two halves of the partial-sum buffer; fewer waves share the chain when

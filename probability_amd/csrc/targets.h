@@ -120,7 +120,42 @@ __device__ inline void put_slot(T (&v)[K], int s, T f) {
 constexpr int kBijAny = 0;            // chosen at run time from the layout
 constexpr int kBijSlot0Softplus = 1;  // every bijector lane: Softplus, slot 0
 
-template <typename T, int K, int FAM>
+// Shared LDS of a chain group (G chains of one GLM problem whose owner
+// waves take turns with all W waves of the block streaming X once for every
+// chain): the chains' coefficients beta [G][K*64] and noise scales [G], the
+// owners' done flags [G], and per wave and chain the partial gradient
+// [W][G][K*64], log-likelihood [W][G] (float64) and residual sum of squares
+// [W][G].
+template <typename T, int K>
+struct GroupLayout {
+  static __host__ __device__ constexpr size_t al(size_t n) {
+    return (n + 15) & ~(size_t)15;
+  }
+  static __host__ __device__ constexpr size_t beta(int) { return 0; }
+  static __host__ __device__ constexpr size_t sig(int g) {
+    return al((size_t)g * K * 64 * sizeof(T));
+  }
+  static __host__ __device__ constexpr size_t done(int g) {
+    return sig(g) + al((size_t)g * sizeof(T));
+  }
+  static __host__ __device__ constexpr size_t red(int g) {
+    return done(g) + al((size_t)g * sizeof(int));
+  }
+  static __host__ __device__ constexpr size_t lp(int g, int w) {
+    return red(g) + al((size_t)w * g * K * 64 * sizeof(T));
+  }
+  static __host__ __device__ constexpr size_t ssq(int g, int w) {
+    return lp(g, w) + al((size_t)w * g * sizeof(double));
+  }
+  static __host__ __device__ constexpr size_t bytes(int g, int w) {
+    return ssq(g, w) + al((size_t)w * g * sizeof(T));
+  }
+};
+
+// G > 1: the GLM target of a chain group (nuts_group_kernel); every
+// evaluation is a round in which all waves of the block pass over X for all
+// G chains.
+template <typename T, int K, int FAM, int G = 1>
 struct Target {
   int kind, dim, n_obs;
   T p0, p1;
@@ -147,6 +182,10 @@ struct Target {
   // turn: a wave can only reuse a half after every wave has read it)
   int split_w = 0, split_n = 1, red_half = 0;
   lptr<unsigned char> red = nullptr;
+  // chain group: the chain this wave owns (-1: none), this wave and the
+  // block's waves, and the group's shared LDS (GroupLayout)
+  int grp_j = -1, grp_w = 0, grp_waves = 1;
+  lptr<unsigned char> grp = nullptr;
 
   __device__ void load(const hipmc_problem& pb, T* wave_lds) {
     const int lane = lane_id();
@@ -212,6 +251,13 @@ struct Target {
     split_w = w;
     split_n = n;
     red = (lptr<unsigned char>)buf;
+  }
+
+  __device__ void set_group(int j, int w, int waves, unsigned char* buf) {
+    grp_j = j;
+    grp_w = w;
+    grp_waves = waves;
+    grp = (lptr<unsigned char>)buf;
   }
 
   // Per-lane partial log-prob (double) and wave-uniform part `lpu`; writes
@@ -683,6 +729,9 @@ struct Target {
   // added in a fixed order.
   template <bool LINEAR>
   __device__ double glm(const T (&th)[K], T (&g)[K], double& lpu) {
+    if constexpr (G > 1) {
+      return glm_grouped<LINEAR>(th, g, lpu);
+    }
     using VT = typename Vec16<T>::type;
     constexpr int V = Vec16<T>::n;
     constexpr int CPB = 16 * V;                 // coefficients per block
@@ -803,6 +852,275 @@ struct Target {
     }
     if (LINEAR) {
       const T S = wave_sum(ssq);
+      const T ss = p1;
+      const T gs = S / (sigma * sigma * sigma) - (T)n / sigma - sigma / (ss * ss);
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        if (k == (dx >> 6) && lane == (dx & 63)) g[k] = gs;
+      lp += -(double)n * (double)log(sigma) +
+            (double)(T(-0.5) * (sigma / ss) * (sigma / ss));
+      if (sigma < T(0)) lp = neg_inf();
+    }
+    lpu = lp;
+    return lpl;
+  }
+
+  // ---------------------------------------------------------- chain groups
+  // Block-wide barrier with LDS ordering; every wave of a group block
+  // reaches the same number of them (two per round).
+  __device__ static void group_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  }
+
+  // The pass over this wave's rows for all G chains (as glm_pass, with B = 2
+  // rows per phase per batch): logits, residuals and gradient partials of
+  // every chain from each loaded row.
+  template <int NBB, bool LINEAR>
+  __device__ void group_pass(int dx, int xp, const int (&cst)[kGlmBlocks<T, K>],
+                             typename Vec16<T>::type (&acc)[G][kGlmBlocks<T, K>],
+                             double (&lpl)[G], T (&ssq)[G]) {
+    using VT = typename Vec16<T>::type;
+    constexpr int V = Vec16<T>::n;
+    constexpr int CPB = 16 * V;
+    constexpr int B = 2;
+    constexpr int RB = 4 * B;
+    constexpr int CH = 64 * V;
+    constexpr int NQ = CH / RB;
+    const int lane = lane_id();
+    const int ph = lane >> 4, grp16 = lane & 15;
+    const int n = n_obs;
+    const bool counts = grp16 < B;
+    lptr<const T> beta = (lptr<const T>)(grp + GroupLayout<T, K>::beta(G));
+    lptr<const T> sg = (lptr<const T>)(grp + GroupLayout<T, K>::sig(G));
+    VT bs[G][NBB];
+    T inv_s2[G];
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+#pragma unroll
+      for (int b = 0; b < NBB; ++b)
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+          const int c = b * CPB + V * grp16 + v;
+          bs[j][b][v] = c < dx ? beta[j * K * 64 + c] : T(0);
+        }
+      const T s = LINEAR ? sg[j] : T(1);
+      inv_s2[j] = T(1) / (s * s);
+    }
+    const int q0 = grp_w * CH, qs = grp_waves * CH;
+    auto row0 = [&](int q) { return q0 + (q / NQ) * qs + (q % NQ) * RB; };
+    auto load = [&](int r0, VT (&xv)[B][NBB], T& yv) {
+#pragma unroll
+      for (int u = 0; u < B; ++u) {
+        const int j = r0 + 4 * u + ph;
+        const int jl = j < n ? j : n - 1;
+#pragma unroll
+        for (int b = 0; b < NBB; ++b)
+          xv[u][b] = *(gptr<VT>)(x + (size_t)jl * xp + cst[b]);
+      }
+      const int jr = r0 + 4 * (lane & 1) + ph;
+      yv = y[jr < n ? jr : n - 1];
+    };
+    int r0 = row0(0);
+    if (r0 >= n) return;
+    VT xv[B][NBB];
+    T yv;
+    load(r0, xv, yv);
+    for (int q = 1;; ++q) {
+      const int r1 = row0(q);
+      const bool more = r1 < n;
+      VT nx[B][NBB];
+      T ny;
+      if (more) load(r1, nx, ny);
+      const bool valid = r0 + 4 * (lane & 1) + ph < n;
+      const bool add = valid && counts;
+#pragma unroll
+      for (int j = 0; j < G; ++j) {
+        T sr[B];
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+          T a = T(0);
+#pragma unroll
+          for (int b = 0; b < NBB; ++b)
+#pragma unroll
+            for (int v = 0; v < V; ++v) a += xv[u][b][v] * bs[j][b][v];
+          sr[u] = row_sum(a);
+        }
+        const T s = (lane & 1) ? sr[1] : sr[0];
+        T r;
+        if (LINEAR) {
+          const T res = yv - s;
+          r = res * inv_s2[j];
+          lpl[j] += add ? (double)(T(-0.5) * res * res * inv_s2[j]) : 0.0;
+          ssq[j] += add ? res * res : T(0);
+        } else {
+          const T w0 = T(1) - yv;
+          const SoftplusParts<T> sp = softplus_parts(s);
+          const T l0 = w0 == T(0) ? T(0) : -(fmax(s, T(0)) + sp.l) * w0;
+          const T l1 = yv == T(0) ? T(0) : -(fmax(-s, T(0)) + sp.l) * yv;
+          lpl[j] += add ? (double)(l0 + l1) : 0.0;
+          r = yv - sp.sig;
+        }
+        r = valid ? r : T(0);
+#pragma unroll
+        for (int u = 0; u < B; ++u) {
+          const T rj = quad_bcast(r, u);
+#pragma unroll
+          for (int b = 0; b < NBB; ++b) acc[j][b] += xv[u][b] * rj;
+        }
+      }
+      if (!more) break;
+      r0 = r1;
+#pragma unroll
+      for (int u = 0; u < B; ++u)
+#pragma unroll
+        for (int b = 0; b < NBB; ++b) xv[u][b] = nx[u][b];
+      yv = ny;
+    }
+  }
+
+  // One round: every wave passes over its rows for all G chains and leaves
+  // its partial sums in LDS.  Returns false (after the first barrier) once
+  // every owner has set its done flag.
+  template <bool LINEAR>
+  __device__ bool group_round() {
+    using VT = typename Vec16<T>::type;
+    constexpr int V = Vec16<T>::n;
+    constexpr int CPB = 16 * V;
+    constexpr int NB = kGlmBlocks<T, K>;
+    group_barrier();  // 1: coefficients and flags are written
+    lptr<const int> done = (lptr<const int>)(grp + GroupLayout<T, K>::done(G));
+    bool all = true;
+#pragma unroll
+    for (int j = 0; j < G; ++j) all = all && done[j] != 0;
+    if (all) return false;
+    const int lane = lane_id();
+    const int dx = LINEAR ? dim - 1 : dim;
+    const int xp = x_pitch;
+    const int grp16 = lane & 15;
+    int cst[NB];
+    VT acc[G][NB];
+    double lpl[G];
+    T ssq[G];
+#pragma unroll
+    for (int b = 0; b < NB; ++b) {
+      const int c = b * CPB + V * grp16;
+      cst[b] = c < xp ? c : 0;
+    }
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+#pragma unroll
+      for (int b = 0; b < NB; ++b) acc[j][b] = VT{};
+      lpl[j] = 0.0;
+      ssq[j] = T(0);
+    }
+    const int nbl = (dx + CPB - 1) / CPB;  // live blocks
+    if (nbl <= 1 || NB == 1) {
+      group_pass<1, LINEAR>(dx, xp, cst, acc, lpl, ssq);
+    } else if (nbl == 2 || NB == 2) {
+      group_pass<(NB >= 2 ? 2 : 1), LINEAR>(dx, xp, cst, acc, lpl, ssq);
+    } else {
+      group_pass<NB, LINEAR>(dx, xp, cst, acc, lpl, ssq);
+    }
+    // sum the four row phases (lanes g, g+16, g+32, g+48) and leave this
+    // wave's partial sums in its slots
+    lptr<T> red_g = (lptr<T>)(grp + GroupLayout<T, K>::red(G));
+    lptr<double> red_l = (lptr<double>)(grp + GroupLayout<T, K>::lp(G, grp_waves));
+    lptr<T> red_s = (lptr<T>)(grp + GroupLayout<T, K>::ssq(G, grp_waves));
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+      lptr<T> dst = red_g + ((size_t)grp_w * G + j) * K * 64;
+#pragma unroll
+      for (int b = 0; b < NB; ++b) {
+        VT v = acc[j][b];
+#pragma unroll
+        for (int e = 0; e < V; ++e) {
+          const T s1 = v[e] + __shfl_xor(v[e], 16);
+          v[e] = s1 + __shfl_xor(s1, 32);
+        }
+        if (lane < 16 && b * CPB + V * grp16 < K * 64) {
+#pragma unroll
+          for (int e = 0; e < V; ++e) dst[b * CPB + V * grp16 + e] = v[e];
+        }
+      }
+      const double lw = wave_sum(lpl[j]);
+      const T sw = wave_sum(ssq[j]);
+      if (lane == 0) {
+        red_l[grp_w * G + j] = lw;
+        red_s[grp_w * G + j] = sw;
+      }
+    }
+    group_barrier();  // 2: every wave's partial sums are in LDS
+    return true;
+  }
+
+  __device__ bool group_round_any() {
+    return kind == HIPMC_TARGET_LINEAR ? group_round<true>()
+                                       : group_round<false>();
+  }
+
+  // A wave with nothing (more) to evaluate joins the rounds until every
+  // chain of the group is done; an owner first sets its done flag.
+  __device__ void group_finish() {
+    if (grp_j >= 0) {
+      lptr<int> done = (lptr<int>)(grp + GroupLayout<T, K>::done(G));
+      if (lane_id() == 0) done[grp_j] = 1;
+    }
+    while (group_round_any()) {
+    }
+  }
+
+  // The owner's evaluation: publish the chain's coefficients (and noise
+  // scale), take part in one round, add up the waves' partial sums in a
+  // fixed order, then the prior terms as in glm().
+  template <bool LINEAR>
+  __device__ double glm_grouped(const T (&th)[K], T (&g)[K], double& lpu) {
+    const int lane = lane_id();
+    const int dx = LINEAR ? dim - 1 : dim;
+    const int n = n_obs;
+    T sigma = T(1);
+    if (LINEAR) {
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        if (k == (dx >> 6)) sigma = readlane(th[k], dx & 63);
+    }
+    lptr<T> beta = (lptr<T>)(grp + GroupLayout<T, K>::beta(G)) + grp_j * K * 64;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int e = k * 64 + lane;
+      beta[e] = e < dx ? th[k] : T(0);
+    }
+    if (lane == 0) ((lptr<T>)(grp + GroupLayout<T, K>::sig(G)))[grp_j] = sigma;
+    group_round<LINEAR>();
+    lptr<const T> red_g = (lptr<const T>)(grp + GroupLayout<T, K>::red(G));
+    lptr<const double> red_l =
+        (lptr<const double>)(grp + GroupLayout<T, K>::lp(G, grp_waves));
+    lptr<const T> red_s =
+        (lptr<const T>)(grp + GroupLayout<T, K>::ssq(G, grp_waves));
+    T gl[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) gl[k] = T(0);
+    double ll = 0.0;
+    T S = T(0);
+    for (int w = 0; w < grp_waves; ++w) {
+      lptr<const T> src = red_g + ((size_t)w * G + grp_j) * K * 64;
+#pragma unroll
+      for (int k = 0; k < K; ++k) gl[k] += src[k * 64 + lane];
+      ll += red_l[w * G + grp_j];
+      S += red_s[w * G + grp_j];
+    }
+    double lpl = 0.0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int e = k * 64 + lane;
+      const bool isb = e < dx;
+      const T bv = th[k] * inv_s0;
+      lpl += isb ? (double)(T(-0.5) * bv * bv) : 0.0;
+      g[k] = isb ? gl[k] - th[k] * inv_s0sq : T(0);
+    }
+    double lp = log_norm + ll;
+    if (LINEAR) {
       const T ss = p1;
       const T gs = S / (sigma * sigma * sigma) - (T)n / sigma - sigma / (ss * ss);
 #pragma unroll

@@ -57,10 +57,16 @@ class PosteriorBatch:
     waves_per_chain: GLMs only, waves sharing one chain's gradient (1-8;
       None: chosen from the largest design matrix).  Like slot_dim, equal
       values give bit-identical results alone or in any mix.
+    chains_per_block: GLMs only, chains of one problem that share every
+      pass over X (1, 2 or 4; the kernel uses fewer where the LDS or the
+      waves do not allow more).  Every problem's chain count must be a
+      multiple of it (else 1 is used).  Results depend on it only through
+      the order of floating-point sums; equal values give bit-identical
+      results alone or in any mix.
   """
 
   def __init__(self, specs, num_chains, bijectors=None, keys=None,
-               slot_dim=None, waves_per_chain=None):
+               slot_dim=None, waves_per_chain=None, chains_per_block=1):
     if not specs:
       raise ValueError('need at least one problem')
     self.specs = [targets_lib.as_target(s) for s in specs]
@@ -98,6 +104,12 @@ class PosteriorBatch:
     self.kind = self.specs[0].kind
     if _family(self.kind) == 2 and len({s.kind for s in self.specs}) > 1:
       self.kind = _lib.TARGET_DIAG_NORMAL
+    g = int(chains_per_block or 1)
+    if g not in (1, 2, 4):
+      raise ValueError('chains_per_block must be 1, 2 or 4')
+    if _family(self.kind) != 1 or any(n % g for n in self.num_chains):
+      g = 1
+    self.chains_per_block = g
     if _family(self.kind) != 1:
       self.waves_per_chain = 1
     elif waves_per_chain is not None:
@@ -171,7 +183,8 @@ def sample_posteriors(batch, num_results, num_burnin_steps=0,
 
 
 def sample_ragged(specs, num_chains, num_results, bijectors=None, keys=None,
-                  slot_dim=None, progress=None, stats=None, **kwargs):
+                  slot_dim=None, progress=None, stats=None, chains_per_block=1,
+                  **kwargs):
   """sample_posteriors for problems of very different sizes: the GLMs are
   split into classes by the waves that share one of their chains (from each
   problem's own N x D), every class is its own batch on its own stream, and
@@ -204,9 +217,12 @@ def sample_ragged(specs, num_chains, num_results, bijectors=None, keys=None,
     idx = [i for i in range(p) if cls[i] == w]
     if glm:
       idx = _xcd_order(idx, [specs[i].n_obs * specs[i].dim for i in idx])
+    g = (chains_per_block.get(w, 1) if isinstance(chains_per_block, dict)
+         else chains_per_block)
     batch = PosteriorBatch([specs[i] for i in idx], [chains[i] for i in idx],
                            [bijs[i] for i in idx], [keys[i] for i in idx],
-                           slot_dim=slot_dim, waves_per_chain=w)
+                           slot_dim=slot_dim, waves_per_chain=w,
+                           chains_per_block=g)
     stream = torch.cuda.Stream(device=dev)
     stream.wait_stream(caller)
     sub = {} if stats is not None else None
@@ -502,6 +518,9 @@ class _Run:
     a.unrolled_leapfrog_steps = self.unrolled
     a.target_kind = b.kind
     a.waves_per_chain = b.waves_per_chain
+    # chain groups only for multi-transition launches: a single transition
+    # (an adaptation step) is bound by each group's slowest chain
+    a.chains_per_block = b.chains_per_block if n > 1 else 1
     a.max_energy_diff = float(self.max_energy_diff)
     a.problems = b.table.data_ptr()
     a.n_problems = b.n_problems

@@ -124,3 +124,68 @@ def test_sample_ragged_equals_each_problem_alone(dtype):
     np.testing.assert_array_equal(mix.final_state[k].cpu().numpy(),
                                   alone.final_state[0].cpu().numpy())
     assert float(mix.step_size[k]) == float(alone.step_size[0])
+
+
+def _glm_group_problems(dtype):
+  r = np.random.default_rng(21)
+  specs, bij, oracles = [], [], []
+  for k, (nobs, d) in enumerate(((700, 40), (2700, 100), (300, 9),
+                                 (1500, 60))):
+    x = r.normal(size=(nobs, d)).astype(np.float32) * 0.3
+    beta = r.normal(size=d) * 0.3
+    if k % 2 == 0:
+      y = (r.uniform(size=nobs) < 1 / (1 + np.exp(-(x @ beta)))).astype(
+          np.float32)
+      specs.append(ptg.LogisticRegression(x, y, dtype=dtype))
+      bij.append(None)
+      oracles.append(otg.LogisticRegression(x.astype(np.float64),
+                                            y.astype(np.float64)))
+    else:
+      y = (x @ beta + 0.5 * r.normal(size=nobs)).astype(np.float32)
+      specs.append(ptg.LinearRegression(x, y, dtype=dtype))
+      bij.append([pbij.Identity(), pbij.Exp()])
+      codes = np.zeros(d + 1, np.int32)
+      codes[d] = 2
+      oracles.append(otg.Transformed(
+          otg.LinearRegression(x.astype(np.float64), y.astype(np.float64)),
+          codes))
+  return specs, bij, oracles
+
+
+@pytest.mark.parametrize('group', [2, 4])
+@pytest.mark.parametrize('waves', [4, 8])
+@pytest.mark.parametrize('dtype,ndt,tol', [(torch.float32, np.float32, 2e-4),
+                                           (torch.float64, np.float64, 1e-9)])
+def test_chain_groups_match_oracle(group, waves, dtype, ndt, tol):
+  # chains of one problem sharing every pass over X (nuts_group_kernel):
+  # each chain still follows the oracle's transitions
+  specs, bij, oracles = _glm_group_problems(dtype)
+  chains = 8
+  batch = batched.PosteriorBatch(specs, chains, bij, slot_dim=128,
+                                 waves_per_chain=waves,
+                                 chains_per_block=group)
+  assert batch.chains_per_block == group
+  res = _run(batch, seed=4)
+  for k, ot in enumerate(oracles):
+    d = specs[k].dim
+    ox, otrace, _ = omcmc.sample_nuts_da(
+        ot, np.zeros((chains, d)), np.asarray(0.05, ndt), 6, 5, 6,
+        seed=orng.sanitize_seed(4), target_accept_prob=0.8, dtype=ndt,
+        problem=k)
+    ostates = np.stack([ot.forward(v) if hasattr(ot, 'forward') else v
+                        for v in ox])
+    lf = np.stack([r.leapfrogs_taken for r in otrace])
+    np.testing.assert_array_equal(res.leapfrogs_taken[k].cpu().numpy(), lf)
+    np.testing.assert_allclose(res.draws[k].cpu().numpy(), ostates, rtol=tol,
+                               atol=tol)
+
+
+def test_chain_groups_alone_equals_mix():
+  specs, bij, _ = _glm_group_problems(torch.float32)
+  mk = lambda s, b, keys: batched.PosteriorBatch(
+      s, 8, b, keys=keys, slot_dim=128, waves_per_chain=8, chains_per_block=4)
+  mix = _run(mk(specs, bij, None), seed=8)
+  for k in range(len(specs)):
+    alone = _run(mk([specs[k]], [bij[k]], [k]), seed=8)
+    np.testing.assert_array_equal(mix.draws[k].cpu().numpy(),
+                                  alone.draws[0].cpu().numpy())
