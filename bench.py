@@ -1,28 +1,31 @@
-"""Benchmark: min-ESS/s at split-R-hat < 1.01 for NUTS + dual averaging +
-Softplus TTK on the non-centred hierarchical normal (J = 256, 1024 chains,
-1000 adaptation + 2000 draws, float32) on one MI355X.
+"""Benchmark: min-ESS/s at split-R-hat < 1.01, batched vs one posterior per
+launch, on one MI355X.
 
-One "step" is one full sampling run (bootstrap, 1000 adapted NUTS
-transitions, 2000 recorded transitions) with the data already in HBM.
-`value` = min over the 258 parameters of the cross-chain ESS of the 2000
-draws, divided by the wall time of one run.
-
-Also reported: the plain execution of the same chain (host step loop, one
-NUTS launch + one DA launch per transition) timed once, the NUTS kernel's
-achieved FP32 rate against the MI355X FP32 peak (HIP events over every
-sampling launch), and the CPU restatement of TFP's batched algorithm timed
-on a bounded sample of transitions on one host core.
+Default workload (ragged_glms_512): 512 independent Bayesian GLMs (256
+logistic, 256 Gaussian linear with an Exp bijector on sigma),
+N_k ~ LogUniform[128, 8192], D_k ~ U{4..128}, X iid N(0,1) float32, 32
+chains each, NUTS (max_tree_depth 10) with per-problem windowed adaptation
+(dual-averaging step size and diagonal mass matrix, target 0.85), 500
+adaptation + 1000 draws, the problems batched in size classes on concurrent
+streams with the data in HBM.  One step is 75 transitions of all 512
+problems, so --steps 20 times exactly one complete run; value = the sum of
+min-ESS over the problems that reach split R-hat < 1.01 (the others count
+zero) divided by the timed seconds.  Beside it: the same problems one per
+launch (a stratified sample, timed alone), the GPU kernels' design-matrix
+read rate against HBM bandwidth, and the CPU restatement of TFP's batched
+NUTS timed on the host cores.
 
 Other workloads (one JSON line each, same fields):
+  --workload hier_normal_j256     NUTS+DA+TTK(Softplus), hierarchical normal
+                                  J=256, 1024 chains, 1000 + 2000; one step
+                                  is one sampling run
   --workload logistic_1024x25     plain HMC, logistic regression 1024x25,
                                   64 chains, L=10, 500 + 1000 steps
   --workload logistic_65536x256   HMC + dual averaging on the BF16 MFMA
                                   gradient path, 4096 chains x 256, L=20
-  --workload ragged_glms_512      512 ragged GLMs x 32 chains, NUTS with
-                                  per-problem dual averaging, batched vs one
-                                  posterior per launch
   --workload serving_poisson200   continuous-batching server, Poisson
                                   arrivals at 200 req/s
+  --workload reducers_1024x258    streaming reducers at 1024 x 258
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload W]
 """
@@ -54,9 +57,9 @@ def flops_per_leapfrog(j):
 def parse():
   ap = argparse.ArgumentParser()
   ap.add_argument('--gpus', type=int, default=1)
-  ap.add_argument('--steps', type=int, default=3)
-  ap.add_argument('--warmup', type=int, default=1)
-  ap.add_argument('--cpu-seconds', type=float, default=12.0)
+  ap.add_argument('--steps', type=int, default=20)
+  ap.add_argument('--warmup', type=int, default=5)
+  ap.add_argument('--cpu-seconds', type=float, default=24.0)
   ap.add_argument('--no-cpu', action='store_true')
   ap.add_argument('--no-plain', action='store_true')
   ap.add_argument('--seed', type=int, default=20241)
@@ -66,7 +69,13 @@ def parse():
                   help='override the number of draws (profiling runs)')
   ap.add_argument('--calib-copy', action='store_true',
                   help='run one 256 MiB device copy (counter calibration)')
-  ap.add_argument('--workload', default='hier_normal_j256',
+  ap.add_argument('--c3-dual-averaging', action='store_true',
+                  help='ragged GLMs: step-size adaptation only (no mass '
+                       'matrix)')
+  ap.add_argument('--no-c3-compare', dest='c3_compare', action='store_false',
+                  help='ragged GLMs: skip the dual-averaging-only comparison '
+                       'run')
+  ap.add_argument('--workload', default='ragged_glms_512',
                   choices=['hier_normal_j256', 'logistic_1024x25',
                            'logistic_65536x256', 'ragged_glms_512',
                            'serving_poisson200', 'reducers_1024x258'])
@@ -181,7 +190,7 @@ def main():
     dst.copy_(src)
     torch.cuda.synchronize()
     del src, dst
-  if args.workload != 'hier_normal_j256':
+  if args.workload != 'hier_normal_j256':  # the workloads of bench_workloads
     import bench_workloads
     line = bench_workloads.run(args, torch, pa)
     print(json.dumps(line), flush=True)

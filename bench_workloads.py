@@ -6,11 +6,13 @@
                       steps) + 1000 results on X [65536, 256] stored BF16,
                       4096 chains; the all-chain gradient runs as BF16 MFMA
                       GEMMs with a hi/lo split of beta.
-* ragged_glms_512     512 GLMs (half logistic, half Gaussian linear with
-                      sigma ~ HalfNormal(1) under an Exp bijector),
-                      N_k ~ LogUniform[128, 8192], D_k ~ U{4..128}, 32 chains
-                      each, NUTS + per-problem dual averaging (target 0.8),
-                      500 adaptation + 1000 draws, all problems in shared
+* ragged_glms_512     (the default) 512 GLMs (half logistic, half Gaussian
+                      linear with sigma ~ HalfNormal(1) under an Exp
+                      bijector), N_k ~ LogUniform[128, 8192],
+                      D_k ~ U{4..128}, 32 chains each, NUTS with per-problem
+                      windowed adaptation (step size and diagonal mass
+                      matrix; --c3-dual-averaging: step size only), 500
+                      adaptation + 1000 draws, all problems in shared
                       launches; beside it the same problems one per launch.
 * reducers_1024x258   the streaming reducers at the headline's state shape
                       (1024 chains x 258 parameters, float32 draws):
@@ -243,127 +245,272 @@ def _ragged_specs(torch, pa, n_problems):
   return specs, bij
 
 
+C3_STEP_TRANSITIONS = 75     # one bench step: 1/20 of a 1500-transition run
+# HBM bytes the GLM NUTS kernel moves per design-matrix entry streamed
+# (rocprofv3 FETCH_SIZE + WRITE_SIZE over the shortened C3 run, copy-
+# calibrated; profiles/nuts_kernel_glm_ragged512.md): most X reads hit L2.
+C3_HBM_BYTES_PER_ENTRY = None
+
+
+def _ragged_run(pa, specs, bij, seed, adaptation, na, nr, tap):
+  from probability_amd.mcmc import batched
+  return batched.RaggedRun(
+      specs, 32, nr, bijectors=bij, slot_dim=192, seed=seed,
+      num_burnin_steps=na, num_adaptation_steps=na, step_size=0.05,
+      target_accept_prob=tap, max_tree_depth=10, max_iters_per_launch=100,
+      adaptation=adaptation, stats={'time_kernels': False})
+
+
+def _ragged_gate(torch, pa, draws):
+  """Per-problem (min ESS, max split R-hat); a problem counts toward the
+  metric only at split R-hat < 1.01."""
+  ess, rhat = [], []
+  for dr in draws:
+    e, r = _ess_rhat(torch, pa, dr)
+    ess.append(float(e.min()))
+    rhat.append(float(r.max()))
+  return np.array(ess), np.array(rhat)
+
+
 def ragged_glms_512(args, torch, pa):
+  """512 ragged GLM posteriors (C3), the default workload.  One step is
+  C3_STEP_TRANSITIONS transitions of all 512 problems, so 20 steps are one
+  complete run (500 adaptation + 1000 draws); the timed region holds exactly
+  `steps` steps: complete runs with seeds seed, seed + 1, ... (a last partial
+  run is timed but its draws are not counted).  Warm-up steps run a separate
+  run's first transitions.  value = sum of min-ESS over the problems at
+  split R-hat < 1.01 (failing problems count zero) / timed seconds."""
   from probability_amd import _lib
   from probability_amd.mcmc import batched
   p, chains = 512, 32
   na = 500 if args.adapt is None else args.adapt
   nr = 1000 if args.draws is None else args.draws
+  total = na + nr
+  per_step = max(1, total // 20) if (args.adapt or args.draws) else \
+      C3_STEP_TRANSITIONS
+  adaptation = 'dual_averaging' if args.c3_dual_averaging else 'windowed'
+  tap = 0.8 if adaptation == 'dual_averaging' else 0.85
   specs, bij = _ragged_specs(torch, pa, p)
-  slot = 192
-  # the sampling phase runs 100 transitions per launch so that progress is
-  # visible between launches
-  kw = dict(num_results=nr, num_burnin_steps=na, num_adaptation_steps=na,
-            step_size=0.05, target_accept_prob=0.8, max_tree_depth=10,
-            max_iters_per_launch=100)
-  mark = [0]
-
-  def report(t, n):
-    if t >= mark[0] + 100 or t == n:
-      mark[0] = 0 if t == n else t
-      torch.cuda.synchronize()
-      _say(f'ragged: {t}/{n} transitions')
-
-  if args.glm_waves is None:
-    # size classes: each problem's chains are shared by the waves its own
-    # N x D asks for; the classes run concurrently on their own streams
-    def run(k, stats=None):
-      return batched.sample_ragged(specs, chains, bijectors=bij,
-                                   slot_dim=slot, seed=args.seed + k,
-                                   stats=stats, progress=report, **kw)
-
-    def waves_of(i):
-      return _lib.glm_waves_per_chain(specs[i].n_obs * specs[i].dim)
+  _say(f'ragged: {adaptation} adaptation, {per_step} transitions per step')
+  if args.warmup:
+    wr = _ragged_run(pa, specs, bij, args.seed + 999, adaptation, na, nr, tap)
+    wr.advance_to(min(total, args.warmup * per_step))
+    torch.cuda.synchronize()
+    del wr
+  runs, done_runs = [], []
+  torch.cuda.synchronize()
+  t0 = time.perf_counter()
+  for s in range(args.steps):
+    if not runs or runs[-1][1] >= total:
+      runs.append([_ragged_run(pa, specs, bij, args.seed + len(runs),
+                               adaptation, na, nr, tap), 0])
+    run = runs[-1]
+    run[1] = min(total, run[1] + per_step)
+    run[0].advance_to(run[1])
+    if run[1] >= total:
+      done_runs.append((run[0].finish(), run[0].stats))
+    torch.cuda.synchronize()
+    if (s + 1) % 5 == 0 or s + 1 == args.steps:
+      _say(f'ragged: step {s + 1}/{args.steps}')
+  torch.cuda.synchronize()
+  wall = time.perf_counter() - t0
+  ms = 1000.0 * wall / max(args.steps, 1)
+  _say(f'ragged: timed {wall:.1f} s; diagnostics')
+  gates = [_ragged_gate(torch, pa, res.draws) for res, _ in done_runs]
+  if gates:
+    ess, rhat = gates[-1]
   else:
-    batch = batched.PosteriorBatch(specs, chains, bijectors=bij,
-                                   slot_dim=slot, waves_per_chain=args.glm_waves)
-
-    def run(k, stats=None):
-      return batched.sample_posteriors(batch, seed=args.seed + k, stats=stats,
-                                       progress=report, **kw)
-
-    def waves_of(i):
-      return batch.waves_per_chain
-
-  _say('ragged: batched runs')
-  stats = {'time_kernels': True}
-  res, ms = _timed(torch, lambda k: run(k, stats=stats),
-                   max(1, min(args.steps, 1)), min(args.warmup, 1))
-  _say(f'ragged: batched {ms:.0f} ms; diagnostics')
-  min_ess, rhat_max = [], []
-  for dr in res.draws:
-    e, r = _ess_rhat(torch, pa, dr)
-    min_ess.append(float(e.min()))
-    rhat_max.append(float(r.max()))
-  min_ess = np.array(min_ess)
-  rhat_max = np.array(rhat_max)
-  lf_total = stats['leapfrogs_total']
-  nd = sum(s.n_obs * s.dim for s in specs)
-  # one posterior per launch on a stratified sample of problems (the same
-  # code and keys, so each problem's draws equal its draws in the batch)
+    ess, rhat = np.zeros(p), np.full(p, np.inf)
+  ok = rhat < 1.01
+  counted = float(sum(float(e[r < 1.01].sum()) for e, r in gates))
+  value = counted / wall
+  # one posterior per launch: a stratified sample of problems, each alone
+  # with the same code, keys, size class and adaptation (its draws equal
+  # its draws in the batch)
   idx = np.linspace(0, p - 1, max(1, args.plain_sample)).round().astype(int)
-  plain_ms = []
+  plain_s = []
   for i in idx:
-    _say(f'ragged: problem {i} alone')
-    one = batched.PosteriorBatch([specs[i]], chains, bijectors=[bij[i]],
-                                 keys=[i], slot_dim=slot,
-                                 waves_per_chain=waves_of(i))
-    _, pm = _timed(torch, lambda k: batched.sample_posteriors(
-        one, seed=args.seed + k, progress=report, **kw), 1, 0)
-    plain_ms.append(pm)
-  plain_total_ms = float(np.mean(plain_ms)) * p
-  agg = float(min_ess.sum()) / (ms / 1e3)
-  classes = stats.get('classes', [(p, waves_of(0))])
-  cls_stats = stats.get('class_stats', [stats])
-  line = _base('aggregate min-ESS/sec over 512 ragged GLM posteriors (NUTS+DA, '
-               '32 chains each), batched vs one posterior per launch',
-               agg, 'min-ESS/s', args, ms, 'f32',
-               {'workload': 'ragged_glms_512: 256 logistic + 256 linear '
-                            '(Exp on sigma), N~LogU[128,8192], D~U{4..128}, '
-                            '32 chains, NUTS(depth 10) + per-problem DA '
-                            f'(target 0.8), {na} adaptation + {nr} draws',
-                'problems': p, 'chains_per_problem': chains,
-                'size_classes': [
-                    {'problems': n, 'waves_per_chain': w,
-                     'entries_total': cs['cost_total'],
-                     'entries_adaptation': cs['cost_adaptation'],
-                     'entries_critical_path': cs['cost_critical'],
-                     'entries_critical_path_adaptation':
-                         cs['cost_critical_adaptation'],
-                     'adaptation_kernel_ms': sum(
-                         l[0] for l in cs['launches'] if l[2] == 1),
-                     'sampling_kernel_ms': sum(
-                         l[0] for l in cs['launches'] if l[2] > 1)}
-                    for (n, w), cs in zip(classes, cls_stats)],
-                'sum_n_times_d': nd})
+    one = batched.RaggedRun([specs[i]], chains, nr, bijectors=[bij[i]],
+                            keys=[int(i)], slot_dim=192, seed=args.seed,
+                            num_burnin_steps=na, num_adaptation_steps=na,
+                            step_size=0.05, target_accept_prob=tap,
+                            max_tree_depth=10, max_iters_per_launch=100,
+                            adaptation=adaptation)
+    torch.cuda.synchronize()
+    ta = time.perf_counter()
+    one.advance_to(total)
+    torch.cuda.synchronize()
+    plain_s.append(time.perf_counter() - ta)
+  plain_total = float(np.mean(plain_s)) * p
+  per_run_s = wall / max(len(done_runs), 1) if done_runs else wall
+  line = _base(
+      'min-ESS/sec at split-Rhat<1.01 over 512 ragged GLM posteriors '
+      '(NUTS, per-problem step-size and mass-matrix adaptation, 32 chains '
+      'each), batched vs one posterior per launch',
+      value, 'min-ESS/s', args, ms, 'f32',
+      {'workload': 'ragged_glms_512: 256 logistic + 256 linear (Exp on '
+                   'sigma) GLMs, N~LogU[128,8192], D~U{4..128}, X iid N(0,1) '
+                   'f32, 32 chains each, NUTS(max_tree_depth 10) with '
+                   f'per-problem {adaptation} adaptation (target {tap}), '
+                   f'{na} adaptation + {nr} draws; one step = {per_step} '
+                   'transitions of all 512 problems',
+       'problems': p, 'chains_per_problem': chains,
+       'adaptation': adaptation, 'transitions_per_step': per_step,
+       'complete_runs_timed': len(done_runs),
+       'parallelism': 'single GPU; problems batched in size classes on '
+                      'concurrent streams'})
   line.update({
-      'min_ess_per_problem_min_median': [float(min_ess.min()),
-                                         float(np.median(min_ess))],
-      'rhat_max': float(rhat_max.max()),
-      'rhat_ok_fraction': float((rhat_max < 1.01).mean()),
-      'chain_grad_evals_per_s': lf_total / (ms / 1e3),
-      'plain': {'value': float(min_ess.sum()) / (plain_total_ms / 1e3),
-                'ms_per_step': plain_total_ms,
+      'problems_passing_rhat': int(ok.sum()),
+      'problems_failing_rhat': int((~ok).sum()),
+      'rhat_max': float(rhat.max()),
+      'min_ess_passing_sum': float(ess[ok].sum()),
+      'min_ess_per_problem_min_median': [float(ess.min()),
+                                         float(np.median(ess))],
+      'run_seconds': per_run_s,
+      'plain': {'value': float(ess[ok].sum()) / plain_total,
+                'seconds_per_run': plain_total,
                 'what': f'each problem alone in its own launches '
                         f'({len(idx)} problems timed, mean x 512)'},
-      'batched_over_plain': plain_total_ms / ms,
-      'roofline': _ragged_roofline(stats, specs, lf_total, ms)})
+      'batched_over_plain': plain_total / per_run_s,
+  })
+  if done_runs:
+    line['roofline'] = _ragged_roofline(specs, done_runs[-1][1], per_run_s)
+  if args.c3_compare and adaptation == 'windowed':
+    rr = _ragged_run(pa, specs, bij, args.seed, 'dual_averaging', na, nr, 0.8)
+    torch.cuda.synchronize()
+    ta = time.perf_counter()
+    rr.advance_to(total)
+    res = rr.finish()
+    torch.cuda.synchronize()
+    td = time.perf_counter() - ta
+    e2, r2 = _ragged_gate(torch, pa, res.draws)
+    line['dual_averaging_only'] = {
+        'value': float(e2[r2 < 1.01].sum()) / td, 'seconds': td,
+        'problems_passing_rhat': int((r2 < 1.01).sum()),
+        'rhat_max': float(r2.max())}
+  if not args.no_cpu:
+    line['cpu_baseline'] = _ragged_cpu(args, specs, line, ess, ok)
   return line
 
 
-def _ragged_roofline(stats, specs, lf_total, wall_ms):
-  # algorithmic FLOP: 4 N_k D_k per chain-leapfrog, weighted by each
-  # problem's share of the leapfrogs (per-problem totals), over the whole
-  # run's wall time (the size classes' kernels overlap on their streams)
-  lf = stats['leapfrogs_per_problem']
-  flop = float(sum(4.0 * s.n_obs * s.dim * float(lf[i])
-                   for i, s in enumerate(specs)))
-  achieved = flop / (wall_ms * 1e-3) / 1e12
-  return {'bound': 'mfma', 'achieved': achieved, 'peak': FP32_PEAK_TFLOPS,
-          'unit': 'TFLOP/s', 'frac': achieved / FP32_PEAK_TFLOPS,
-          'traffic': None, 'kernel': 'nuts_kernel<float,3,glm> (all size '
-                                     'classes, wall time)',
-          'chain_leapfrogs': lf_total}
+def _ragged_roofline(specs, stats, run_s):
+  """Design-matrix bytes streamed per second over one run: every
+  chain-leapfrog reads its problem's X once (N_k D_k 4 bytes; the
+  per-chain evaluation), summed over the run's leapfrogs, over the run's
+  wall time (the size classes' kernels overlap on their streams and cover
+  nearly all of it)."""
+  lpp = stats['leapfrogs_per_problem']
+  entries = float(sum(float(lpp[i]) * s.n_obs * s.dim
+                      for i, s in enumerate(specs)))
+  achieved = entries * 4.0 / run_s / 1e9
+  out = {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS,
+         'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
+         'kernel': 'nuts_kernel<float,3,glm> (all size classes)',
+         'what': 'design-matrix bytes read per second: N_k D_k 4 B per '
+                 'chain-leapfrog, summed over the run, over its wall time '
+                 '(about 95% of them hit L2; the kernel is bound by L2 '
+                 'bandwidth and per-wave load latency)',
+         'chain_leapfrogs': float(lpp.sum()),
+         'design_matrix_entries': entries,
+         'traffic': None}
+  if C3_HBM_BYTES_PER_ENTRY is not None:
+    out['traffic'] = entries * C3_HBM_BYTES_PER_ENTRY
+    out['traffic_what'] = ('HBM bytes over the run from FETCH_SIZE + '
+                           'WRITE_SIZE per design-matrix entry, '
+                           'copy-calibrated')
+  return out
+
+
+class _CpuGlm:
+  """The GLM log-density and gradient in NumPy with the design-matrix
+  products in float32 (the CPU restatement's target; oracle.targets holds
+  the float64 one)."""
+
+  def __init__(self, spec_x, spec_y, linear):
+    self.x = np.ascontiguousarray(spec_x, np.float32)
+    self.y = np.asarray(spec_y, np.float32)
+    self.linear = linear
+
+  def log_prob_and_grad(self, q):
+    q = np.asarray(q, np.float64)
+    if self.linear:
+      beta, ls = q[:, :-1], q[:, -1]
+      sig = np.exp(ls)
+      s = (self.x @ beta.T.astype(np.float32)).T.astype(np.float64)
+      res = self.y[None, :] - s
+      n = self.x.shape[0]
+      lp = (-0.5 * (res ** 2).sum(1) / sig ** 2 - n * np.log(sig)
+            - 0.5 * (beta ** 2).sum(1) - 0.5 * sig ** 2 + ls)
+      gb = (res.astype(np.float32) @ self.x).astype(np.float64) / \
+          sig[:, None] ** 2 - beta
+      gs = ((res ** 2).sum(1) / sig ** 3 - n / sig - sig) * sig + 1.0
+      return lp, np.concatenate([gb, gs[:, None]], 1)
+    s = (self.x @ q.T.astype(np.float32)).T.astype(np.float64)
+    lp = (self.y[None] * s - np.logaddexp(0, s)).sum(1) - 0.5 * (q ** 2).sum(1)
+    r = (self.y[None] - 1 / (1 + np.exp(-s))).astype(np.float32)
+    return lp, (r @ self.x).astype(np.float64) - q
+
+
+def _ragged_cpu(args, specs, line, ess, ok):
+  """TFP's batched NUTS restated in NumPy (oracle.mcmc.nuts_one_step over
+  the float32 GLM above), timed on the host: three problems (smallest,
+  median and largest N x D), post-adaptation transitions of their 32
+  chains within a time budget, as the median of 3 repeats after a warm-up,
+  with BLAS on every core of the process's affinity mask and on one.  The
+  per-entry cost (seconds per chain-leapfrog per design-matrix entry) times
+  the GPU run's entries gives the CPU time of the whole run."""
+  import os
+  from threadpoolctl import threadpool_limits
+  from oracle import mcmc as omcmc
+  from oracle import rng as orng
+  cores = len(os.sched_getaffinity(0))
+  sizes = np.array([s.n_obs * s.dim for s in specs])
+  pick = [int(np.argmin(sizes)), int(np.argsort(sizes)[len(sizes) // 2]),
+          int(np.argmax(sizes))]
+  budget = max(1.0, args.cpu_seconds / 6.0)
+
+  def cost(threads):
+    per_entry = []
+    with threadpool_limits(limits=threads):
+      for i in pick:
+        s = specs[i]
+        lin = s.kind == 2
+        tg = _CpuGlm(s.x.cpu().numpy()[:, :s.dim - (1 if lin else 0)],
+                     s.y.cpu().numpy(), lin)
+        q = np.zeros((32, s.dim))
+        t, g = tg.log_prob_and_grad(q)
+        reps = []
+        for rep in range(4):   # first is warm-up
+          lf, n_it, t0 = 0, 0, time.perf_counter()
+          while time.perf_counter() - t0 < budget / 4 or n_it == 0:
+            q, rr = omcmc.nuts_one_step(tg, 0.02, q, t, g,
+                                        orng.fold_in(5, n_it),
+                                        orng.PhiloxDraws(np.float32),
+                                        max_depth=10)
+            t, g = rr.target_log_prob, rr.grads_target_log_prob
+            lf += int(rr.leapfrogs_taken.sum())
+            n_it += 1
+          if rep:
+            reps.append((time.perf_counter() - t0) / max(lf, 1) /
+                        (s.n_obs * s.dim))
+        per_entry.append(float(np.median(reps)))
+    return float(np.mean(per_entry))
+
+  entries = line.get('roofline', {}).get('design_matrix_entries', 0.0)
+  c_all = cost(cores)
+  c_one = cost(1)
+  cpu_s = entries * c_all
+  return {'value': float(ess[ok].sum()) / cpu_s, 'unit': 'min-ESS/s',
+          'cores': cores, 'kind': 'port',
+          'sample': (f'NUTS transitions of 32 chains on 3 of the 512 problems '
+                     f'(N x D = {", ".join(str(int(sizes[i])) for i in pick)}),'
+                     ' median of 3 after a warm-up, NumPy with float32 '
+                     f'design-matrix products on {cores} BLAS threads; per '
+                     'design-matrix entry cost x the GPU run\'s entries = '
+                     f'{cpu_s:.0f} s for the run'),
+          'seconds_per_run': cpu_s,
+          'one_thread': {'value': float(ess[ok].sum()) / (entries * c_one),
+                         'seconds_per_run': entries * c_one, 'cores': 1}}
 
 
 # ---------------------------------------------------------- C4 serving

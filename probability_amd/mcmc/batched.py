@@ -25,7 +25,7 @@ from probability_amd import random as samplers
 from probability_amd import targets as targets_lib
 
 __all__ = ['PosteriorBatch', 'sample_posteriors', 'sample_ragged',
-           'BatchResults', 'WindowState', 'window_sizes']
+           'RaggedRun', 'BatchResults', 'WindowState', 'window_sizes']
 
 BatchResults = collections.namedtuple('BatchResults', [
     'draws', 'step_size', 'log_accept_ratio', 'leapfrogs_taken',
@@ -191,87 +191,130 @@ def sample_ragged(specs, num_chains, num_results, bijectors=None, keys=None,
   the classes advance concurrently, each at its own pace (small problems do
   not wait for the largest ones' adaptation steps).  A problem's draws
   depend only on its own size class, key and chain count, so they equal its
-  draws alone.  Returns BatchResults in the order of `specs`.
+  draws alone.  chains_per_block: an int, or a dict from a class's waves per
+  chain to the chains per block of its sampling launches.  Returns
+  BatchResults in the order of `specs`.
   """
-  if not specs:
-    raise ValueError('need at least one problem')
-  specs = [targets_lib.as_target(s) for s in specs]
-  p = len(specs)
-  # windowed adaptation starts every chain uniform on (-2, 2) from the setup
-  # half of the seed, as windowed_adaptive_nuts does
-  windowed_init = kwargs.get('adaptation') == 'windowed'
-  if windowed_init:
-    setup_seed, kwargs['seed'] = samplers.split_seed(
-        samplers.sanitize_seed(kwargs.get('seed')), 2)
-  chains = ([int(num_chains)] * p if np.isscalar(num_chains)
-            else [int(c) for c in num_chains])
-  bijs = bijectors or [None] * p
-  keys = list(range(p)) if keys is None else [int(k) for k in keys]
-  glm = _family(specs[0].kind) == 1
-  cls = [_lib.glm_waves_per_chain(s.n_obs * s.dim) if glm else 1
-         for s in specs]
-  dev = specs[0].device
-  caller = torch.cuda.current_stream(dev)
-  runs = []
-  for w in sorted(set(cls), reverse=True):
-    idx = [i for i in range(p) if cls[i] == w]
-    if glm:
-      idx = _xcd_order(idx, [specs[i].n_obs * specs[i].dim for i in idx])
-    g = (chains_per_block.get(w, 1) if isinstance(chains_per_block, dict)
-         else chains_per_block)
-    batch = PosteriorBatch([specs[i] for i in idx], [chains[i] for i in idx],
-                           [bijs[i] for i in idx], [keys[i] for i in idx],
-                           slot_dim=slot_dim, waves_per_chain=w,
-                           chains_per_block=g)
-    stream = torch.cuda.Stream(device=dev)
-    stream.wait_stream(caller)
-    sub = {} if stats is not None else None
-    if sub is not None and stats.get('time_kernels'):
-      sub['time_kernels'] = True
-    runs.append((idx, sub, _Run(batch, stream, num_results, stats=sub,
-                                **kwargs)))
-  total = sum(r.n_total for _, _, r in runs)
-  if windowed_init:
-    for _, _, r in runs:
-      with torch.cuda.stream(r.stream):
-        r.x.copy_(_init_uniform(r.batch, setup_seed))
-        r.tlp, r.grad = r.batch.eval(r.x)
-  while not all(r.done() for _, _, r in runs):
-    for _, _, r in runs:
+  run = RaggedRun(specs, num_chains, num_results, bijectors=bijectors,
+                  keys=keys, slot_dim=slot_dim, stats=stats,
+                  chains_per_block=chains_per_block, **kwargs)
+  while not run.done():
+    run.advance_launches()
+    if progress is not None:
+      progress(run.t, run.n_total)
+  return run.finish()
+
+
+class RaggedRun:
+  """The size classes of sample_ragged, each a batch on its own stream,
+  advanced launch by launch (advance_launches) or up to a transition
+  (advance_to)."""
+
+  def __init__(self, specs, num_chains, num_results, bijectors=None,
+               keys=None, slot_dim=None, stats=None, chains_per_block=1,
+               **kwargs):
+    if not specs:
+      raise ValueError('need at least one problem')
+    specs = [targets_lib.as_target(s) for s in specs]
+    p = len(specs)
+    self.specs, self.stats, self.kwargs = specs, stats, kwargs
+    # windowed adaptation starts every chain uniform on (-2, 2) from the
+    # setup half of the seed, as windowed_adaptive_nuts does
+    windowed_init = kwargs.get('adaptation') == 'windowed'
+    if windowed_init:
+      setup_seed, kwargs['seed'] = samplers.split_seed(
+          samplers.sanitize_seed(kwargs.get('seed')), 2)
+    chains = ([int(num_chains)] * p if np.isscalar(num_chains)
+              else [int(c) for c in num_chains])
+    bijs = bijectors or [None] * p
+    keys = list(range(p)) if keys is None else [int(k) for k in keys]
+    glm = _family(specs[0].kind) == 1
+    cls = [_lib.glm_waves_per_chain(s.n_obs * s.dim) if glm else 1
+           for s in specs]
+    dev = specs[0].device
+    self.caller = torch.cuda.current_stream(dev)
+    self.runs = []
+    for w in sorted(set(cls), reverse=True):
+      idx = [i for i in range(p) if cls[i] == w]
+      if glm:
+        idx = _xcd_order(idx, [specs[i].n_obs * specs[i].dim for i in idx])
+      g = (chains_per_block.get(w, 1) if isinstance(chains_per_block, dict)
+           else chains_per_block)
+      batch = PosteriorBatch([specs[i] for i in idx],
+                             [chains[i] for i in idx],
+                             [bijs[i] for i in idx], [keys[i] for i in idx],
+                             slot_dim=slot_dim, waves_per_chain=w,
+                             chains_per_block=g)
+      stream = torch.cuda.Stream(device=dev)
+      stream.wait_stream(self.caller)
+      sub = {} if stats is not None else None
+      if sub is not None and stats.get('time_kernels'):
+        sub['time_kernels'] = True
+      self.runs.append((idx, sub, _Run(batch, stream, num_results, stats=sub,
+                                       **kwargs)))
+    self.n_total = sum(r.n_total for _, _, r in self.runs)
+    if windowed_init:
+      for _, _, r in self.runs:
+        with torch.cuda.stream(r.stream):
+          r.x.copy_(_init_uniform(r.batch, setup_seed))
+          r.tlp, r.grad = r.batch.eval(r.x)
+
+  @property
+  def t(self):
+    return sum(r.t for _, _, r in self.runs)
+
+  def done(self):
+    return all(r.done() for _, _, r in self.runs)
+
+  def advance_launches(self):
+    """One launch (with its updates) for every class not yet done."""
+    for _, _, r in self.runs:
       if not r.done():
         r.advance()
-    if progress is not None:
-      progress(sum(r.t for _, _, r in runs), total)
-  parts = [(idx, sub, r.finish(consumer=caller)) for idx, sub, r in runs]
-  for _, _, r in runs:
-    caller.wait_stream(r.stream)
-  out = {f: [None] * p for f in BatchResults._fields}
-  step = torch.empty(p, dtype=specs[0].dtype, device=dev)
-  for idx, _, res in parts:
-    for f in BatchResults._fields:
-      if f == 'step_size':
-        step[torch.as_tensor(idx, device=dev)] = res.step_size
-      elif getattr(res, f) is not None:
-        for j, i in enumerate(idx):
-          out[f][i] = getattr(res, f)[j]
-  if stats is not None:
-    stats['leapfrogs_total'] = sum(sub['leapfrogs_total'] for _, sub, _ in parts)
-    lpp = np.zeros(p)
-    for idx, sub, _ in parts:
-      lpp[idx] = sub['leapfrogs_per_problem']
-    stats['leapfrogs_per_problem'] = lpp
-    stats['classes'] = [(len(idx), r.batch.waves_per_chain)
-                        for idx, _, r in runs]
-    stats['class_stats'] = [{k: sub[k] for k in (
-        'cost_total', 'cost_adaptation', 'cost_critical',
-        'cost_critical_adaptation', 'leapfrogs_total', 'launches')
-                             if k in sub} for _, sub, _ in parts]
-    if stats.get('time_kernels'):
-      stats['launches'] = [l for _, sub, _ in parts for l in sub['launches']]
-  out['step_size'] = step
-  if not kwargs.get('trace', True):
-    out['draws'] = None
-  return BatchResults(**out)
+
+  def advance_to(self, t):
+    """Enqueues every class up to transition t (launches end there)."""
+    while any(r.t < min(t, r.n_total) for _, _, r in self.runs):
+      for _, _, r in self.runs:
+        if r.t < min(t, r.n_total):
+          r.advance(limit=t)
+
+  def finish(self):
+    specs, stats, caller = self.specs, self.stats, self.caller
+    p = len(specs)
+    dev = specs[0].device
+    runs = self.runs
+    parts = [(idx, sub, r.finish(consumer=caller)) for idx, sub, r in runs]
+    for _, _, r in runs:
+      caller.wait_stream(r.stream)
+    out = {f: [None] * p for f in BatchResults._fields}
+    step = torch.empty(p, dtype=specs[0].dtype, device=dev)
+    for idx, _, res in parts:
+      for f in BatchResults._fields:
+        if f == 'step_size':
+          step[torch.as_tensor(idx, device=dev)] = res.step_size
+        elif getattr(res, f) is not None:
+          for j, i in enumerate(idx):
+            out[f][i] = getattr(res, f)[j]
+    if stats is not None:
+      stats['leapfrogs_total'] = sum(sub['leapfrogs_total']
+                                     for _, sub, _ in parts)
+      lpp = np.zeros(p)
+      for idx, sub, _ in parts:
+        lpp[idx] = sub['leapfrogs_per_problem']
+      stats['leapfrogs_per_problem'] = lpp
+      stats['classes'] = [(len(idx), r.batch.waves_per_chain)
+                          for idx, _, r in runs]
+      stats['class_stats'] = [{k: sub[k] for k in (
+          'cost_total', 'cost_adaptation', 'cost_critical',
+          'cost_critical_adaptation', 'leapfrogs_total', 'launches')
+                               if k in sub} for _, sub, _ in parts]
+      if stats.get('time_kernels'):
+        stats['launches'] = [l for _, sub, _ in parts for l in sub['launches']]
+    out['step_size'] = step
+    if not self.kwargs.get('trace', True):
+      out['draws'] = None
+    return BatchResults(**out)
 
 
 def _init_uniform(batch, seed):
@@ -426,12 +469,15 @@ class _Run:
   def done(self):
     return self.t >= self.n_total
 
-  def advance(self):
-    """Enqueues the next launch (and its step-size / mass-matrix updates)."""
+  def advance(self, limit=None):
+    """Enqueues the next launch (and its step-size / mass-matrix updates);
+    a launch ends at transition `limit` if given."""
     with torch.cuda.stream(self.stream):
       t = self.t
       adapting = t < self.num_adapt
       n = 1 if adapting else min(self.n_total - t, self.max_iters)
+      if limit is not None:
+        n = max(1, min(n, limit - t))
       if self.windowed and adapting:
         self._window_pre(t)
       if self.history:
