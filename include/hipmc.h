@@ -79,6 +79,7 @@ extern "C" {
 #define HIPMC_TARGET_DIAG_NORMAL 4  /* independent N(loc_i, scale_i) */
 #define HIPMC_TARGET_FUNNEL 5       /* Neal's funnel [v, x] */
 #define HIPMC_TARGET_NAN_EXP 6      /* sum(where(x<0, NaN, -x)) */
+#define HIPMC_TARGET_LOG_GAMMA 7    /* x = log Gamma(param[0], rate param[1]) */
 
 /* bijector codes */
 #define HIPMC_BIJ_IDENTITY 0
@@ -106,6 +107,8 @@ extern "C" {
  *                      param[0] = mu prior scale, param[1] = tau scale.
  *   DIAG_NORMAL:       y = loc [dim], aux = scale [dim].
  *   FUNNEL / NAN_EXP:  no data.
+ *   LOG_GAMMA:         iid elements; param[0] = concentration,
+ *                      param[1] = rate.
  * Bijector ranges [off, off+len) over the flat state, code per range.
  */
 typedef struct hipmc_problem {

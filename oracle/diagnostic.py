@@ -29,6 +29,12 @@ def auto_covariance(x, axis=0):
 def effective_sample_size(states, cross_chain_dims=1, filter_threshold=0.0):
   """states [N, C, ...] -> ESS [...] (cross-chain when cross_chain_dims=1)."""
   states = np.asarray(states, np.float64)
+  if cross_chain_dims is not None and not isinstance(cross_chain_dims, int):
+    # several chain axes: the same statistic over their flattened product
+    dims = [int(d) for d in cross_chain_dims]
+    rest = [d for d in range(1, states.ndim) if d not in dims]
+    x = np.transpose(states, [0] + dims + rest)
+    states = x.reshape((x.shape[0], -1) + tuple(states.shape[d] for d in rest))
   n = states.shape[0]
   auto_cov = auto_covariance(states, axis=0)
   if cross_chain_dims is not None:

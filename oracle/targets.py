@@ -169,6 +169,25 @@ class NealsFunnel(Target):
     return tlp, np.concatenate([g_v[:, None], g_x], axis=1)
 
 
+class LogGamma(Target):
+  """x_i = log of iid Gamma(concentration, rate): the log-gamma target of
+  `mcmc/hmc_test.py:146-200` (written there with TransformedDistribution(
+  Gamma, Log)); density c x - r e^x + c log r - lgamma(c) per element."""
+
+  def __init__(self, dim, concentration, rate):
+    self.part_sizes = (dim,)
+    self.c, self.r = float(concentration), float(rate)
+
+  def log_prob_and_grad(self, x):
+    import math
+    x = np.asarray(x, np.float64)
+    with np.errstate(over='ignore'):
+      ex = np.exp(x)
+      tlp = (self.c * x - self.r * ex).sum(-1) + x.shape[-1] * (
+          self.c * math.log(self.r) - math.lgamma(self.c))
+      return tlp, self.c - self.r * ex
+
+
 class NaNExponential(Target):
   """log p = sum(where(x < 0, NaN, -x)): the NaN-rejection target of
   `mcmc/hmc_test.py:398-431`."""

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(_HERE, 'libhipmc.so')
 F32, F64 = 0, 1
 TARGET_LOGISTIC, TARGET_LINEAR, TARGET_HIER_NORMAL = 1, 2, 3
 TARGET_DIAG_NORMAL, TARGET_FUNNEL, TARGET_NAN_EXP = 4, 5, 6
+TARGET_LOG_GAMMA = 7
 BIJ_IDENTITY, BIJ_SOFTPLUS, BIJ_EXP = 0, 1, 2
 MAX_BIJ_RANGES = 4
 

@@ -85,7 +85,7 @@ int check_run(const hipmc_run_args* a, bool nuts) {
 // Calls L<T, FAM>::run(args...) for the launch's dtype and target family.
 template <template <typename, int> class L, typename... A>
 int by_family(int dtype, int kind, A&&... args) {
-  if (kind < HIPMC_TARGET_LOGISTIC || kind > HIPMC_TARGET_NAN_EXP)
+  if (kind < HIPMC_TARGET_LOGISTIC || kind > HIPMC_TARGET_LOG_GAMMA)
     return fail(HIPMC_E_INVALID, "unknown target kind");
   const int fam = family_of(kind);
   if (dtype == HIPMC_F32) {

@@ -319,6 +319,7 @@ struct Target {
       switch (kind) {
         case HIPMC_TARGET_DIAG_NORMAL: lpl = diag_normal(th, g, lpu); break;
         case HIPMC_TARGET_FUNNEL: lpl = funnel(th, g, lpu); break;
+        case HIPMC_TARGET_LOG_GAMMA: lpl = log_gamma(th, g, lpu); break;
         default: lpl = nan_exp(th, g, lpu); break;
       }
     }
@@ -436,6 +437,24 @@ struct Target {
     lpu = log_norm + (double)(T(-0.5) * (v / T(3)) * (v / T(3))) -
           (double)(T(0.5) * nk * v) - (double)(T(0.5) * S * inv_s2);
     return 0.0;
+  }
+
+  // x_i = log of a Gamma(concentration p0, rate p1) variable:
+  // log p = p0 x - p1 exp(x) (+ constants in log_norm), gradient
+  // p0 - p1 exp(x) (the log-gamma target of mcmc/hmc_test.py:146-200).
+  __device__ double log_gamma(const T (&th)[K], T (&g)[K], double& lpu) {
+    const int lane = lane_id();
+    double lpl = 0.0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const int e = k * 64 + lane;
+      const bool v = e < dim;
+      const T ex = exp(th[k]);
+      lpl += v ? (double)(p0 * th[k] - p1 * ex) : 0.0;
+      g[k] = v ? p0 - p1 * ex : T(0);
+    }
+    lpu = log_norm;
+    return lpl;
   }
 
   __device__ double nan_exp(const T (&th)[K], T (&g)[K], double& lpu) {

@@ -33,8 +33,23 @@ def auto_covariance(x):
   return ac / (n - k).reshape((-1,) + (1,) * (x.dim() - 1))
 
 
+def _flatten_chain_dims(states, cross_chain_dims):
+  """Moves the chain dimensions (an int or a sequence of ints, counted in
+  `states`) to axis 1 as one flattened chain axis."""
+  dims = ([cross_chain_dims] if isinstance(cross_chain_dims, int)
+          else [int(d) for d in cross_chain_dims])
+  dims = [d % states.dim() for d in dims]
+  if 0 in dims:
+    raise ValueError('cross_chain_dims cannot include the sample axis 0')
+  rest = [d for d in range(1, states.dim()) if d not in dims]
+  x = states.permute([0] + dims + rest)
+  return x.reshape((x.shape[0], -1) + tuple(states.shape[d] for d in rest))
+
+
 def _ess_one(states, cross_chain_dims, filter_threshold):
   states = _as64(states)
+  if cross_chain_dims is not None:
+    states = _flatten_chain_dims(states, cross_chain_dims)
   n = states.shape[0]
   ac = auto_covariance(states)
   if cross_chain_dims is not None:
@@ -76,6 +91,12 @@ def effective_sample_size(states, filter_threshold=0., filter_beyond_lag=None,
 def _rhat_one(state, independent_chain_ndims, split_chains):
   x = _as64(state)
   icn = independent_chain_ndims
+  if icn < 1:
+    raise ValueError('Argument `independent_chain_ndims` must be `>= 1`, '
+                     'found: {}'.format(icn))
+  if not split_chains and x.shape[0] < 2:
+    raise ValueError('Must provide at least 2 samples. Found {}'.format(
+        x.shape[0]))
   if split_chains:
     n = x.shape[0]
     if n < 4:

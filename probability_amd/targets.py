@@ -423,6 +423,27 @@ class NaNExponential(TargetSpec):
     p.log_norm = 0.0
 
 
+class LogGamma(TargetSpec):
+  """x_i = log of iid Gamma(concentration, rate) variables; one part of
+  shape [D]: log p = sum(concentration x - rate exp(x)) + D (concentration
+  log(rate) - lgamma(concentration)).  E[x] = digamma(concentration) -
+  log(rate), E[exp(x)] = concentration / rate (`mcmc/hmc_test.py:146-200`).
+  """
+
+  kind = _lib.TARGET_LOG_GAMMA
+
+  def __init__(self, dim, concentration, rate, dtype=torch.float32,
+               device=None):
+    super().__init__(dtype, device)
+    self.event_shapes = [(int(dim),)]
+    self.concentration, self.rate = float(concentration), float(rate)
+
+  def _fill(self, p):
+    c, r = self.concentration, self.rate
+    p.param[0], p.param[1] = c, r
+    p.log_norm = self.dim * (c * math.log(r) - math.lgamma(c))
+
+
 def as_target(fn):
   """Returns `fn` if it is a device target spec, else raises TypeError."""
   if isinstance(fn, TargetSpec):

@@ -30,6 +30,18 @@ def _np(t):
   return t.detach().cpu().numpy()
 
 
+def _err(label, got, want):
+  """Prints the largest absolute error of got vs want and that error over
+  the largest |want| (normwise relative; run with -s to collect them)."""
+  got, want = np.asarray(got, np.float64), np.asarray(want, np.float64)
+  fin = np.isfinite(want)
+  if not fin.any():
+    return
+  d = np.abs(got[fin] - want[fin]).max()
+  rel = d / max(np.abs(want[fin]).max(), 1e-300)
+  print(f'ERR {label} abs={d:.3e} rel={rel:.3e}', flush=True)
+
+
 def _case(kind, dtype_np, seed=0, c=64):
   """(oracle target, product spec ctor, initial state [C, D])."""
   r = np.random.default_rng(seed)
@@ -128,6 +140,11 @@ def _case(kind, dtype_np, seed=0, c=64):
     mk = lambda dt: ptg.NealsFunnel(10, dtype=dt)
     q = cast(r.normal(size=(c, 10)))
     return o, mk, q, None
+  if kind == 'loggamma':
+    o = otg.LogGamma(5, 5.0, 10.0)
+    mk = lambda dt: ptg.LogGamma(5, 5.0, 10.0, dtype=dt)
+    q = cast(-0.8 + 0.4 * r.normal(size=(c, 5)))
+    return o, mk, q, None
   raise ValueError(kind)
 
 
@@ -186,7 +203,8 @@ def test_draws_match_oracle(tdt, ndt, tol):
 # ---------------------------------------------------------------- targets
 @pytest.mark.parametrize('tdt,ndt,tol', DTYPES)
 @pytest.mark.parametrize('kind', ['logistic', 'linear', 'hier', 'diag',
-                                  'funnel', 'logistic_split', 'linear_split',
+                                  'funnel', 'loggamma', 'logistic_split',
+                                  'linear_split',
                                   'logistic_wide', 'logistic_d150',
                                   'linear_wide', 'logistic_tall'])
 def test_target_eval_matches_oracle(kind, tdt, ndt, tol):
@@ -199,6 +217,9 @@ def test_target_eval_matches_oracle(kind, tdt, ndt, tol):
   np.testing.assert_allclose(_np(tlp), rt, rtol=tol * 5, atol=tol * 5)
   scale = np.abs(rg).max()
   np.testing.assert_allclose(_np(grad), rg, rtol=tol * 10, atol=tol * 10 * scale)
+  lab = f'eval {kind} {np.dtype(ndt).name}'
+  _err(lab + ' tlp', _np(tlp), rt)
+  _err(lab + ' grad', _np(grad), rg)
 
 
 @pytest.mark.parametrize('tdt,ndt,tol', DTYPES)
@@ -215,13 +236,14 @@ def test_target_callable_like_tfp(tdt, ndt, tol):
 # ---------------------------------------------------------------- HMC
 @pytest.mark.parametrize('tdt,ndt,tol', DTYPES)
 @pytest.mark.parametrize('kind', ['logistic', 'hier', 'logistic_split',
-                                  'logistic_wide'])
+                                  'logistic_wide', 'loggamma'])
 def test_hmc_one_step_matches_oracle(kind, tdt, ndt, tol):
   o, mk, q, codes = _case(kind, ndt, seed=3)
   spec = mk(tdt)
   ot = _otarget(o, codes)
   eps, nl = {'logistic': (0.05, 10), 'hier': (0.2, 6),
-             'logistic_split': (0.05, 8), 'logistic_wide': (0.02, 8)}[kind]
+             'logistic_split': (0.05, 8), 'logistic_wide': (0.02, 8),
+             'loggamma': (0.3, 4)}[kind]
   kern = pmcmc.HamiltonianMonteCarlo(
       spec if codes is None else ptg.TransformedTarget(spec, _bijs(spec, codes)),
       step_size=eps, num_leapfrog_steps=nl)
@@ -240,6 +262,11 @@ def test_hmc_one_step_matches_oracle(kind, tdt, ndt, tol):
                              rtol=tol * 100, atol=tol * 100)
   np.testing.assert_allclose(_np(spec.pack_state(res.proposed_state)),
                              ores.proposed_state, rtol=tol * 10, atol=tol * 10)
+  lab = f'hmc {kind} {np.dtype(ndt).name}'
+  _err(lab + ' state', _np(spec.pack_state(nxt)), oq)
+  _err(lab + ' proposed', _np(spec.pack_state(res.proposed_state)),
+       ores.proposed_state)
+  _err(lab + ' lar', _np(res.log_accept_ratio), ores.log_accept_ratio)
 
 
 @pytest.mark.parametrize('tdt,ndt,tol', DTYPES)
@@ -279,7 +306,7 @@ def _nuts_one(kind, tdt, ndt, seed=0, c=64, eps=None, max_depth=10):
                 'funnel': 0.9, 'logistic_split': 0.04,
                 'linear_split': 0.02, 'logistic_wide': 0.02,
                 'logistic_d150': 0.05, 'linear_wide': 0.01,
-                'logistic_tall': 0.03}[kind]
+                'logistic_tall': 0.03, 'loggamma': 0.3}[kind]
   tspec = spec if codes is None else ptg.TransformedTarget(spec,
                                                            _bijs(spec, codes))
   kern = pmcmc.NoUTurnSampler(tspec, step_size=np.asarray(eps, ndt),
@@ -298,7 +325,8 @@ def _nuts_one(kind, tdt, ndt, seed=0, c=64, eps=None, max_depth=10):
 
 @pytest.mark.parametrize('tdt,ndt,tol', DTYPES)
 @pytest.mark.parametrize('kind', ['hier', 'hier256', 'logistic', 'linear',
-                                  'diag', 'funnel', 'logistic_split',
+                                  'diag', 'funnel', 'loggamma',
+                                  'logistic_split',
                                   'linear_split', 'logistic_wide',
                                   'logistic_d150', 'linear_wide',
                                   'logistic_tall'])
@@ -321,6 +349,14 @@ def test_nuts_one_step_matches_oracle(kind, tdt, ndt, tol):
                                atol=ftol * 10)
     np.testing.assert_allclose(_np(res.target_log_prob), ores.target_log_prob,
                                rtol=ftol, atol=ftol)
+    np.testing.assert_allclose(_np(res.energy), ores.energy, rtol=ftol,
+                               atol=ftol)
+    lab = f'nuts {kind} {np.dtype(ndt).name} seed{seed}'
+    _err(lab + ' state', _np(spec.pack_state(nxt)), oq)
+    _err(lab + ' tlp', _np(res.target_log_prob), ores.target_log_prob)
+    _err(lab + ' energy', _np(res.energy), ores.energy)
+    _err(lab + ' lar', _np(res.log_accept_ratio)[fin],
+         ores.log_accept_ratio[fin])
 
 
 def test_nuts_divergence_on_gpu_funnel():

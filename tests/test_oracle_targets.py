@@ -102,3 +102,25 @@ def test_softplus_bijector_round_trip_and_fldj():
   np.testing.assert_allclose(bij.fldj(bij.SOFTPLUS, x), np.log(bij.sigmoid(x)),
                              rtol=1e-12, atol=1e-12)
   np.testing.assert_allclose(bij.inverse(bij.EXP, bij.forward(bij.EXP, x)), x)
+
+
+def test_log_gamma_density_gradient_and_moments():
+  # log of a Gamma(c, rate r) variable: scipy's loggamma is the log of a
+  # Gamma(c, 1) variable, so x + log r follows it.
+  t = tg.LogGamma(4, 5.0, 10.0)
+  x = _rng(3).normal(-0.7, 0.5, size=(6, 4))
+  lp, g = t.log_prob_and_grad(x)
+  ref = stats.loggamma.logpdf(x + np.log(10.0), 5.0).sum(-1)
+  np.testing.assert_allclose(lp, ref, rtol=1e-12)
+  np.testing.assert_allclose(g, tg.finite_difference_grad(t, x, h=1e-6),
+                             rtol=1e-6, atol=1e-6)
+  # E[x] = digamma(c) - log r and E[exp x] = c / r, by quadrature of the
+  # oracle's own density (the expectations hmc_test.py checks by sampling).
+  from scipy import integrate, special
+  dens = lambda v: np.exp(tg.LogGamma(1, 5.0, 10.0).log_prob_and_grad(
+      np.array([[v]]))[0][0])
+  assert abs(integrate.quad(dens, -12, 6)[0] - 1) < 1e-9
+  ex = integrate.quad(lambda v: v * dens(v), -12, 6)[0]
+  assert abs(ex - (special.digamma(5.0) - np.log(10.0))) < 1e-9
+  eex = integrate.quad(lambda v: np.exp(v) * dens(v), -12, 6)[0]
+  assert abs(eex - 0.5) < 1e-9
