@@ -42,6 +42,21 @@ def _err(label, got, want):
   print(f'ERR {label} abs={d:.3e} rel={rel:.3e}', flush=True)
 
 
+def _check(label, got, want, tol, scale=None):
+  """Normwise bound: max |got - want| <= tol * max(scale, 1), scale
+  defaulting to the largest finite |want|; prints the measured error."""
+  got, want = np.asarray(got, np.float64), np.asarray(want, np.float64)
+  fin = np.isfinite(want)
+  np.testing.assert_array_equal(np.isfinite(got), fin)
+  _err(label, got, want)
+  if not fin.any():
+    return
+  if scale is None:
+    scale = np.abs(want[fin]).max()
+  d = np.abs(got[fin] - want[fin]).max()
+  assert d <= tol * max(scale, 1.0), (label, d, tol * max(scale, 1.0))
+
+
 def _case(kind, dtype_np, seed=0, c=64):
   """(oracle target, product spec ctor, initial state [C, D])."""
   r = np.random.default_rng(seed)
@@ -214,12 +229,9 @@ def test_target_eval_matches_oracle(kind, tdt, ndt, tol):
   x = torch.as_tensor(q, dtype=tdt, device='cuda')
   tlp, grad = spec.log_prob_and_grad_flat(x, _bijs(spec, codes))
   rt, rg = ot.log_prob_and_grad(q)
-  np.testing.assert_allclose(_np(tlp), rt, rtol=tol * 5, atol=tol * 5)
-  scale = np.abs(rg).max()
-  np.testing.assert_allclose(_np(grad), rg, rtol=tol * 10, atol=tol * 10 * scale)
   lab = f'eval {kind} {np.dtype(ndt).name}'
-  _err(lab + ' tlp', _np(tlp), rt)
-  _err(lab + ' grad', _np(grad), rg)
+  _check(lab + ' tlp', _np(tlp), rt, tol)
+  _check(lab + ' grad', _np(grad), rg, tol)
 
 
 @pytest.mark.parametrize('tdt,ndt,tol', DTYPES)
@@ -256,17 +268,14 @@ def test_hmc_one_step_matches_oracle(kind, tdt, ndt, tol):
   oq, ores = omcmc.hmc_one_step(ot, np.asarray(eps, ndt), nl, q, t0, g0, seed,
                                 orng.PhiloxDraws(ndt))
   np.testing.assert_array_equal(_np(res.is_accepted), ores.is_accepted)
-  np.testing.assert_allclose(_np(spec.pack_state(nxt)), oq, rtol=tol * 10,
-                             atol=tol * 10)
-  np.testing.assert_allclose(_np(res.log_accept_ratio), ores.log_accept_ratio,
-                             rtol=tol * 100, atol=tol * 100)
-  np.testing.assert_allclose(_np(spec.pack_state(res.proposed_state)),
-                             ores.proposed_state, rtol=tol * 10, atol=tol * 10)
   lab = f'hmc {kind} {np.dtype(ndt).name}'
-  _err(lab + ' state', _np(spec.pack_state(nxt)), oq)
-  _err(lab + ' proposed', _np(spec.pack_state(res.proposed_state)),
-       ores.proposed_state)
-  _err(lab + ' lar', _np(res.log_accept_ratio), ores.log_accept_ratio)
+  _check(lab + ' state', _np(spec.pack_state(nxt)), oq, tol)
+  _check(lab + ' proposed', _np(spec.pack_state(res.proposed_state)),
+         ores.proposed_state, tol)
+  # the log accept ratio is a difference of energies: its error is bounded
+  # relative to the target log-prob's magnitude
+  _check(lab + ' lar', _np(res.log_accept_ratio), ores.log_accept_ratio, tol,
+         scale=np.abs(t0).max())
 
 
 @pytest.mark.parametrize('tdt,ndt,tol', DTYPES)
@@ -340,23 +349,12 @@ def test_nuts_one_step_matches_oracle(kind, tdt, ndt, tol):
                                   ores.reach_max_depth)
     np.testing.assert_array_equal(_np(res.has_divergence),
                                   ores.has_divergence)
-    ftol = tol * 20
-    np.testing.assert_allclose(_np(spec.pack_state(nxt)), oq, rtol=ftol,
-                               atol=ftol)
-    fin = np.isfinite(ores.log_accept_ratio)
-    np.testing.assert_allclose(_np(res.log_accept_ratio)[fin],
-                               ores.log_accept_ratio[fin], rtol=ftol * 10,
-                               atol=ftol * 10)
-    np.testing.assert_allclose(_np(res.target_log_prob), ores.target_log_prob,
-                               rtol=ftol, atol=ftol)
-    np.testing.assert_allclose(_np(res.energy), ores.energy, rtol=ftol,
-                               atol=ftol)
     lab = f'nuts {kind} {np.dtype(ndt).name} seed{seed}'
-    _err(lab + ' state', _np(spec.pack_state(nxt)), oq)
-    _err(lab + ' tlp', _np(res.target_log_prob), ores.target_log_prob)
-    _err(lab + ' energy', _np(res.energy), ores.energy)
-    _err(lab + ' lar', _np(res.log_accept_ratio)[fin],
-         ores.log_accept_ratio[fin])
+    _check(lab + ' state', _np(spec.pack_state(nxt)), oq, tol)
+    _check(lab + ' tlp', _np(res.target_log_prob), ores.target_log_prob, tol)
+    _check(lab + ' energy', _np(res.energy), ores.energy, tol)
+    _check(lab + ' lar', _np(res.log_accept_ratio), ores.log_accept_ratio,
+           tol, scale=np.abs(ores.energy[np.isfinite(ores.energy)]).max())
 
 
 def test_nuts_divergence_on_gpu_funnel():
