@@ -522,25 +522,20 @@ def serving_poisson200(args, torch, pa):
   from probability_amd import workloads
   n_req = args.requests
   rate = 200.0
-  r = np.random.default_rng(11)
-  arrivals = np.cumsum(r.exponential(1.0 / rate, size=n_req))
-  glms = workloads.ragged_glms(n_req, seed=13)
+  arrivals, models = workloads.serving_requests(n_req, seed=11, rate=rate)
   cfgs = []
-  for k in range(n_req):
-    if k % 2 == 0:
-      prob = glms[k]
-      if prob['kind'] == 'logistic':
-        spec = pa.targets.LogisticRegression(prob['x'], prob['y'],
-                                             device='cuda')
-        bij = None
-      else:
-        spec = pa.targets.LinearRegression(prob['x'], prob['y'],
+  for prob in models:
+    if prob['kind'] == 'logistic':
+      spec = pa.targets.LogisticRegression(prob['x'], prob['y'],
                                            device='cuda')
-        bij = [pa.bijectors.Identity(), pa.bijectors.Exp()]
+      bij = None
+    elif prob['kind'] == 'linear':
+      spec = pa.targets.LinearRegression(prob['x'], prob['y'],
+                                         device='cuda')
+      bij = [pa.bijectors.Identity(), pa.bijectors.Exp()]
     else:
-      j = int(r.integers(8, 511))
-      y, sig = workloads.hier_normal(j, seed=1000 + k)
-      spec = pa.targets.HierarchicalNormal(y, sig, device='cuda')
+      spec = pa.targets.HierarchicalNormal(prob['y'], prob['sigma'],
+                                           device='cuda')
       bij = [pa.bijectors.Identity(), pa.bijectors.Softplus(),
              pa.bijectors.Identity()]
     cfgs.append((spec, bij))
@@ -584,7 +579,7 @@ def serving_poisson200(args, torch, pa):
                n_req / wall, 'req/s', args, 1000.0 * wall, 'f32',
                {'workload': f'serving_poisson200: {n_req} requests, half '
                             'ragged GLMs (N~LogU[128,8192], D~U{4..128}), '
-                            'half hierarchical normals (J~U{8..510}, Softplus '
+                            'half hierarchical normals (J~U{8..1024}, Softplus '
                             'on tau); 16 chains, 500 adaptation + 2000 draws',
                 'requests': n_req, 'arrival_rate': rate,
                 'hbm_budget_gib': args.hbm_budget_gib})

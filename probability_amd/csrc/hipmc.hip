@@ -40,12 +40,16 @@ int hip_check(hipError_t e, const char* what) {
   return fail(HIPMC_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
-int pick_k(int max_dim) {
-  static const int ks[] = {1, 2, 3, 4, 5, 8};
-  for (int k : ks)
+int pick_k(int max_dim, int fam) {
+  static const int ks[] = {1, 2, 3, 4, 5, 8, 12, 17};
+  for (int k : ks) {
+    if (k > max_slots(fam)) break;
     if (k * 64 >= max_dim) return k;
+  }
   return -1;
 }
+
+int max_state_dim(int fam) { return 64 * max_slots(fam); }
 
 }  // namespace hipmc
 
@@ -61,8 +65,15 @@ int check_run(const hipmc_run_args* a, bool nuts) {
     return fail(HIPMC_E_INVALID, "n_iters > 1 requires fold_seed");
   if (a->max_dim < 1 || a->max_dim > a->dim_stride)
     return fail(HIPMC_E_INVALID, "need 1 <= max_dim <= dim_stride");
-  if (pick_k(a->max_dim) < 0)
-    return fail(HIPMC_E_UNSUPPORTED, "state dimension above 512 is not built");
+  {
+    const int kind = a->target_kind;
+    const int fam = (kind >= HIPMC_TARGET_LOGISTIC && kind <= HIPMC_TARGET_LOG_GAMMA)
+                        ? family_of(kind) : kFamilyTest;
+    if (pick_k(a->max_dim, fam) < 0)
+      return fail(HIPMC_E_UNSUPPORTED,
+                  "state dimension above " + std::to_string(max_state_dim(fam)) +
+                      " is not built for this target family");
+  }
   if (!a->problems || a->n_problems < 1)
     return fail(HIPMC_E_INVALID, "problems is NULL");
   if (!a->step_size || !a->state || !a->tlp || !a->grad)
@@ -104,13 +115,13 @@ int by_family(int dtype, int kind, A&&... args) {
 template <typename T, int FAM>
 struct NutsL {
   static int run(const hipmc_run_args& a, hipStream_t s) {
-    return launch_nuts_k<T, FAM>(pick_k(a.max_dim), a, s);
+    return launch_nuts_k<T, FAM>(pick_k(a.max_dim, FAM), a, s);
   }
 };
 template <typename T, int FAM>
 struct HmcL {
   static int run(bool calibrated, const hipmc_run_args& a, hipStream_t s) {
-    return launch_hmc_k<T, FAM>(pick_k(a.max_dim), calibrated, a, s);
+    return launch_hmc_k<T, FAM>(pick_k(a.max_dim, FAM), calibrated, a, s);
   }
 };
 template <typename T, int FAM>
@@ -624,8 +635,14 @@ int hipmc_target_eval(int32_t dtype, int32_t target_kind,
   if (!problems || !x) return fail(HIPMC_E_INVALID, "problems/x is NULL");
   if (max_dim < 1 || max_dim > dim_stride)
     return fail(HIPMC_E_INVALID, "need 1 <= max_dim <= dim_stride");
-  const int k = pick_k(max_dim);
-  if (k < 0) return fail(HIPMC_E_UNSUPPORTED, "state dimension above 512 is not built");
+  if (target_kind < HIPMC_TARGET_LOGISTIC || target_kind > HIPMC_TARGET_LOG_GAMMA)
+    return fail(HIPMC_E_INVALID, "unknown target kind");
+  const int fam = family_of(target_kind);
+  const int k = pick_k(max_dim, fam);
+  if (k < 0)
+    return fail(HIPMC_E_UNSUPPORTED,
+                "state dimension above " + std::to_string(max_state_dim(fam)) +
+                    " is not built for this target family");
   return by_family<EvalL>(dtype, target_kind, k, problems, chain_problem,
                          n_chains, dim_stride, x, out_tlp, out_grad,
                          as_stream(stream));

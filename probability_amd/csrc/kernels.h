@@ -302,11 +302,16 @@ __host__ __device__ inline int waves_per_chain(const hipmc_run_args& a) {
 // in LDS for the whole leaf loop (kTreeSlots slots, one copy per chain); the
 // leaf's momentum, momentum sum and subtree candidate are parked around
 // each gradient evaluation (kLeafSlots slots per wave).  The other families
-// keep everything in registers (one wave per SIMD).
+// keep everything in registers (one wave per SIMD), except float32 states
+// wider than 512 elements, which park the same way (at 17 slots and depth 10
+// the U-turn memory, parked vectors and target scratch take 157 KiB of the
+// workgroup's 160 KiB); float64 ones could not hold both in LDS.
 constexpr int kTreeSlots = 11;
 constexpr int kLeafSlots = 4;
-template <int FAM>
-constexpr bool parks() { return FAM == kFamilyGlm; }
+template <typename T, int K, int FAM>
+constexpr bool parks() {
+  return FAM == kFamilyGlm || (K > 8 && sizeof(T) == 4);
+}
 
 // LDS of one chain (elements of T): U-turn memory velocity [max_depth][K*64]
 // and cumulative momentum [max_depth][K*64], the parked tree vectors
@@ -315,12 +320,12 @@ constexpr bool parks() { return FAM == kFamilyGlm; }
 template <typename T, int K, int FAM>
 __host__ __device__ constexpr size_t nuts_chain_lds_elems(int max_depth) {
   return (size_t)2 * max_depth * K * 64 +
-         (parks<FAM>() ? (size_t)kTreeSlots * K * 64 : 0);
+         (parks<T, K, FAM>() ? (size_t)kTreeSlots * K * 64 : 0);
 }
 template <typename T, int K, int FAM>
 __host__ __device__ constexpr size_t nuts_wave_lds_elems() {
   return target_scratch_elems<T, K, FAM>() +
-         (parks<FAM>() ? (size_t)kLeafSlots * K * 64 : 0);
+         (parks<T, K, FAM>() ? (size_t)kLeafSlots * K * 64 : 0);
 }
 // LDS of one wave when every wave holds its own chain.
 template <typename T, int K, int FAM>
@@ -345,7 +350,7 @@ __device__ __attribute__((always_inline)) inline void nuts_chain(
     const hipmc_run_args& a, Target<T, K, FAM, G>& tg, const ChainId& id,
     int c, int wic, int W, T* memP, T* memC, T* tree, T* leafp,
     const hipmc_problem& pb) {
-  constexpr bool PARK = parks<FAM>();
+  constexpr bool PARK = parks<T, K, FAM>();
   constexpr size_t S = (size_t)K * 64;
   const int lane = lane_id();
   const int max_depth = a.max_tree_depth;

@@ -93,8 +93,19 @@ int launch_nuts(const hipmc_run_args& a0, hipStream_t s) {
     return hip_check(hipGetLastError(), "nuts_kernel launch");
   }
   const size_t per_wave = (cl + wl) * sizeof(T);
-  if (per_wave > (size_t)kMaxLdsPerBlock)
-    return fail(HIPMC_E_UNSUPPORTED, "U-turn memory does not fit in LDS");
+  if (per_wave > (size_t)kMaxLdsPerBlock) {
+    // wide states at deep trees: one chain per block with up to the
+    // workgroup's whole LDS
+    if (per_wave > (size_t)kMaxLdsSharedChain)
+      return fail(HIPMC_E_UNSUPPORTED,
+                  "U-turn memory does not fit in LDS (lower max_tree_depth "
+                  "or use float32)");
+    const int rc = allow_lds(&nuts_kernel<T, K, FAM>, per_wave);
+    if (rc != HIPMC_OK) return rc;
+    hipLaunchKernelGGL((nuts_kernel<T, K, FAM>), dim3(a.n_chains), dim3(64),
+                       per_wave, s, a);
+    return hip_check(hipGetLastError(), "nuts_kernel launch");
+  }
   const int w = waves_for_lds(per_wave);
   const int blocks = (a.n_chains + w - 1) / w;
   hipLaunchKernelGGL((nuts_kernel<T, K, FAM>), dim3(blocks), dim3(64 * w),
@@ -144,9 +155,14 @@ int launch_eval(const hipmc_problem* p, const int32_t* cp, int n, int stride,
   return hip_check(hipGetLastError(), "eval_kernel launch");
 }
 
-// The slot counts built for every (dtype, family) pair; pick_k (hipmc.hip)
-// chooses the smallest that holds the state.
+// The slot counts built for every (dtype, family) pair, and the wide ones
+// built for the families other than GLMs; pick_k (hipmc.hip) chooses the
+// smallest that holds the state.
 #define HIPMC_FOR_EACH_K(X) X(1) X(2) X(3) X(4) X(5) X(8)
+#define HIPMC_FOR_EACH_WIDE_K(X) X(12) X(17)
+static_assert(max_slots(kFamilyGlm) == 8 && max_slots(kFamilyHier) == 17 &&
+                  max_slots(kFamilyTest) == 17,
+              "slot lists and max_slots disagree");
 
 template <typename T, int FAM>
 int launch_nuts_k(int k, const hipmc_run_args& a, hipStream_t s) {
@@ -154,8 +170,11 @@ int launch_nuts_k(int k, const hipmc_run_args& a, hipStream_t s) {
 #define HIPMC_CASE(KK) \
   case KK: return launch_nuts<T, KK, FAM>(a, s);
     HIPMC_FOR_EACH_K(HIPMC_CASE)
-#undef HIPMC_CASE
   }
+  if constexpr (FAM != kFamilyGlm) {
+    switch (k) { HIPMC_FOR_EACH_WIDE_K(HIPMC_CASE) }
+  }
+#undef HIPMC_CASE
   return fail(HIPMC_E_UNSUPPORTED, "no kernel for this dimension");
 }
 
@@ -168,8 +187,11 @@ int launch_hmc_k(int k, bool calibrated, const hipmc_run_args& a,
     return calibrated ? launch_hmc<T, KK, FAM, true>(a, s)    \
                       : launch_hmc<T, KK, FAM, false>(a, s);
     HIPMC_FOR_EACH_K(HIPMC_CASE)
-#undef HIPMC_CASE
   }
+  if constexpr (FAM != kFamilyGlm) {
+    switch (k) { HIPMC_FOR_EACH_WIDE_K(HIPMC_CASE) }
+  }
+#undef HIPMC_CASE
   return fail(HIPMC_E_UNSUPPORTED, "no kernel for this dimension");
 }
 
@@ -181,8 +203,11 @@ int launch_eval_k(int k, const hipmc_problem* p, const int32_t* cp, int n,
 #define HIPMC_CASE(KK) \
   case KK: return launch_eval<T, KK, FAM>(p, cp, n, stride, x, tlp, grad, s);
     HIPMC_FOR_EACH_K(HIPMC_CASE)
-#undef HIPMC_CASE
   }
+  if constexpr (FAM != kFamilyGlm) {
+    switch (k) { HIPMC_FOR_EACH_WIDE_K(HIPMC_CASE) }
+  }
+#undef HIPMC_CASE
   return fail(HIPMC_E_UNSUPPORTED, "no kernel for this dimension");
 }
 

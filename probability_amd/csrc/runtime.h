@@ -28,8 +28,14 @@ constexpr int kMaxLdsPerBlock = 64 * 1024;
 // more than 64 KiB raises the kernel's dynamic-LDS limit first.
 constexpr int kMaxLdsSharedChain = 160 * 1024;
 
-// Smallest compiled slot count K with K*64 >= max_dim, or -1.
-int pick_k(int max_dim);
+// Register slots per lane (state elements / 64) built for a target family:
+// up to 8 for GLMs, whose chains keep a second copy of their vectors parked
+// in LDS, and up to 17 (1088 elements) for the others.
+constexpr int max_slots(int fam) { return fam == 1 ? 8 : 17; }
+// Smallest compiled slot count K of family fam with K*64 >= max_dim, or -1.
+int pick_k(int max_dim, int fam);
+// The largest state dimension built for family fam.
+int max_state_dim(int fam);
 
 // Sampling-kernel launchers of one (dtype T, family FAM) pair; k is the
 // register slot count from pick_k.

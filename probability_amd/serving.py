@@ -65,6 +65,7 @@ class Request:
     self.da_slot = None
     self.in_flight = False
     self.waves = None
+    self.pool_key = None
 
   @property
   def total_steps(self):
@@ -140,12 +141,13 @@ class _Pool:
 
 
 class _Lane:
-  """The launches of one (family, waves per chain, phase): a stream and the
-  launch in flight (completion event, its requests, buffers kept alive until
-  it ends)."""
+  """The launches of one (slot class, waves per chain, phase): a stream and
+  the launch in flight (completion event, its requests, buffers kept alive
+  until it ends).  pool_key is the slot class: (family, state slots)."""
 
-  def __init__(self, family, waves, adapting, device):
-    self.family = family
+  def __init__(self, pool_key, waves, adapting, device):
+    self.pool_key = pool_key
+    self.family = pool_key[0]
     self.waves = waves
     self.adapting = adapting
     self.stream = torch.cuda.Stream(device=device)
@@ -159,8 +161,11 @@ class SamplerServer:
     hbm_budget_bytes: device bytes the server may hold for admitted requests
       (their data and draw buffers).
     iters_per_tick: transitions a sampling-phase request makes per launch.
-    slot_dims: register-slot class per family ({'glm': 192, 'hier': 512});
-      every request of a family runs the same kernel code.
+    slot_dims: register-slot classes per family, an int or a list of ints
+      ({'glm': 192, 'hier': [512, 1088]}); a request joins the smallest
+      class of its family that holds its state, and the requests of a class
+      run the same kernel code.  Hierarchical states up to 1088 elements
+      (J <= 1086 groups) and GLM states up to 512 are built.
     glm_waves_per_chain: waves sharing each GLM chain's gradient; None (the
       default) takes each request's own from its N x D, a number fixes it
       for every GLM request.  Either way a request's draws do not depend on
@@ -179,9 +184,12 @@ class SamplerServer:
     self.iters_per_tick = int(iters_per_tick)
     self.glm_waves_per_chain = (None if glm_waves_per_chain is None
                                 else int(glm_waves_per_chain))
-    slot_dims = slot_dims or {'glm': 192, 'hier': 512}
-    self.pools = {f: _Pool(f, slot_dims[f], max_chains, max_requests, dtype,
-                           self.device) for f in slot_dims}
+    slot_dims = slot_dims or {'glm': 192, 'hier': [512, 1088]}
+    self.pools = {}
+    for f, dims in slot_dims.items():
+      for d in sorted([dims] if isinstance(dims, int) else dims):
+        self.pools[(f, int(d))] = _Pool(f, int(d), max_chains, max_requests,
+                                        dtype, self.device)
     self.queue = collections.deque()
     self.active = []
     self.finished = []
@@ -195,11 +203,19 @@ class SamplerServer:
       return self.glm_waves_per_chain
     return _lib.glm_waves_per_chain(req.spec.n_obs * req.spec.dim)
 
+  def _pool_key(self, req):
+    fam = _family(req.spec)
+    fits = [k for k in self.pools if k[0] == fam and k[1] >= req.spec.dim]
+    if not fits:
+      raise ValueError(f'no {fam} slot class holds a state of '
+                       f'{req.spec.dim} elements')
+    return min(fits, key=lambda k: k[1])
+
   def _lanes_for(self, req):
-    fam, w = _family(req.spec), req.waves
+    key, w = req.pool_key, req.waves
     for ad in (True, False):
-      if (fam, w, ad) not in self.lanes:
-        self.lanes[(fam, w, ad)] = _Lane(fam, w, ad, self.device)
+      if (key, w, ad) not in self.lanes:
+        self.lanes[(key, w, ad)] = _Lane(key, w, ad, self.device)
 
   # ------------------------------------------------------------ admission
   def submit(self, req):
@@ -211,7 +227,8 @@ class SamplerServer:
     admitted = []
     while self.queue:
       req = self.queue[0]
-      pool = self.pools[_family(req.spec)]
+      req.pool_key = self._pool_key(req)
+      pool = self.pools[req.pool_key]
       need = req.nbytes(torch.tensor([], dtype=self.dtype).element_size())
       if self.used + need > self.budget or not pool.can_hold(req):
         break
@@ -368,12 +385,12 @@ class SamplerServer:
       busy[0].inflight[0].synchronize()
 
   def _start(self, lane):
-    reqs = [r for r in self.active if _family(r.spec) == lane.family and
+    reqs = [r for r in self.active if r.pool_key == lane.pool_key and
             r.waves == lane.waves and r.adapting == lane.adapting and
             not r.in_flight]
     if not reqs:
       return False
-    pool = self.pools[lane.family]
+    pool = self.pools[lane.pool_key]
     keep = self._launch(pool, reqs, lane.adapting, lane.stream)
     ev = torch.cuda.Event()
     ev.record(lane.stream)
@@ -397,7 +414,7 @@ class SamplerServer:
       if r.iter >= r.total_steps:
         done.append(r)
     for r in done:
-      pool = self.pools[_family(r.spec)]
+      pool = self.pools[r.pool_key]
       r.final_step_size = float(pool.step[r.da_slot])
       pool.release(r)
       self.active.remove(r)

@@ -16,6 +16,9 @@
 * ragged_glms            512 GLMs (half logistic, half Gaussian linear with
                          sigma ~ HalfNormal(1)), N_k ~ LogUniform[128, 8192],
                          D_k ~ U{4..128}, 32 chains each.
+* serving_requests       Poisson arrivals at 200 requests/s, alternately a
+                         GLM of ragged_glms's size distribution and a
+                         hierarchical normal with J ~ U{8..1024}.
 """
 
 import math
@@ -105,3 +108,23 @@ def ragged_glms(n_problems=512, seed=3, n_min=128, n_max=8192, d_min=4,
           np.float32)
       out.append(dict(kind='linear', x=x, y=y))
   return out
+
+
+def serving_requests(n_requests, seed=11, rate=200.0, j_min=8, j_max=1024):
+  """Arrival times (seconds, a Poisson process of `rate` per second) and
+  models of the serving workload: even requests the GLMs of
+  ragged_glms(n_requests, seed + 2) (logistic in the first half, linear in
+  the second), odd ones a hierarchical normal (hier_normal) with
+  J ~ U{j_min..j_max}."""
+  r = np.random.default_rng(seed)
+  arrivals = np.cumsum(r.exponential(1.0 / rate, size=n_requests))
+  glms = ragged_glms(n_requests, seed=seed + 2)
+  models = []
+  for k in range(n_requests):
+    if k % 2 == 0:
+      models.append(glms[k])
+    else:
+      j = int(r.integers(j_min, j_max + 1))
+      y, sigma = hier_normal(j, seed=1000 + k)
+      models.append(dict(kind='hier', y=y, sigma=sigma))
+  return arrivals, models

@@ -44,9 +44,17 @@ def test_invalid_arguments_return_codes_without_device():
   assert lib.hipmc_hmc_run(ctypes.byref(a), 0) == _lib.E_INVALID
   a.dtype = 0
   a.n_iters = 1
+  # GLM states are built up to 512 elements, the others up to 1088
+  a.target_kind = _lib.TARGET_LOGISTIC
   a.max_dim = 600
   a.dim_stride = 600
   assert lib.hipmc_nuts_run(ctypes.byref(a), 0) == _lib.E_UNSUPPORTED
+  assert b'above 512' in lib.hipmc_last_error()
+  a.target_kind = _lib.TARGET_HIER_NORMAL
+  assert lib.hipmc_nuts_run(ctypes.byref(a), 0) == _lib.E_INVALID  # no table
+  a.max_dim = a.dim_stride = 1089
+  assert lib.hipmc_nuts_run(ctypes.byref(a), 0) == _lib.E_UNSUPPORTED
+  assert b'above 1088' in lib.hipmc_last_error()
   with pytest.raises(ValueError):
     _lib.check(lib.hipmc_hmc_run(None, 0))
 

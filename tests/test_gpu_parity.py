@@ -132,8 +132,9 @@ def _case(kind, dtype_np, seed=0, c=64):
     codes = np.zeros(7, np.int32)
     codes[6] = 2  # Exp on sigma
     return o, mk, q, codes
-  if kind in ('hier', 'hier256'):
-    j = 70 if kind == 'hier' else 256  # 256: the headline's groups
+  if kind in ('hier', 'hier256', 'hier600', 'hier1024'):
+    # 256: the headline's groups; 600 and 1024: states of 12 and 17 slots
+    j = {'hier': 70, 'hier256': 256, 'hier600': 600, 'hier1024': 1024}[kind]
     y = cast(4 + 3 * r.normal(size=j) + r.uniform(5, 20, j) * r.normal(size=j))
     s = cast(r.uniform(5, 20, j))
     o = otg.HierarchicalNormal(y, s)
@@ -221,7 +222,8 @@ def test_draws_match_oracle(tdt, ndt, tol):
                                   'funnel', 'loggamma', 'logistic_split',
                                   'linear_split',
                                   'logistic_wide', 'logistic_d150',
-                                  'linear_wide', 'logistic_tall'])
+                                  'linear_wide', 'logistic_tall', 'hier600',
+                                  'hier1024'])
 def test_target_eval_matches_oracle(kind, tdt, ndt, tol):
   o, mk, q, codes = _case(kind, ndt)
   spec = mk(tdt)
@@ -248,14 +250,14 @@ def test_target_callable_like_tfp(tdt, ndt, tol):
 # ---------------------------------------------------------------- HMC
 @pytest.mark.parametrize('tdt,ndt,tol', DTYPES)
 @pytest.mark.parametrize('kind', ['logistic', 'hier', 'logistic_split',
-                                  'logistic_wide', 'loggamma'])
+                                  'logistic_wide', 'loggamma', 'hier1024'])
 def test_hmc_one_step_matches_oracle(kind, tdt, ndt, tol):
   o, mk, q, codes = _case(kind, ndt, seed=3)
   spec = mk(tdt)
   ot = _otarget(o, codes)
   eps, nl = {'logistic': (0.05, 10), 'hier': (0.2, 6),
              'logistic_split': (0.05, 8), 'logistic_wide': (0.02, 8),
-             'loggamma': (0.3, 4)}[kind]
+             'loggamma': (0.3, 4), 'hier1024': (0.1, 5)}[kind]
   kern = pmcmc.HamiltonianMonteCarlo(
       spec if codes is None else ptg.TransformedTarget(spec, _bijs(spec, codes)),
       step_size=eps, num_leapfrog_steps=nl)
@@ -311,7 +313,7 @@ def _nuts_one(kind, tdt, ndt, seed=0, c=64, eps=None, max_depth=10):
   spec = mk(tdt)
   ot = _otarget(o, codes)
   eps = eps or {'logistic': 0.08, 'linear': 0.05, 'hier': 0.3,
-                'hier256': 0.3, 'diag': 0.5,
+                'hier256': 0.3, 'hier600': 0.2, 'hier1024': 0.15, 'diag': 0.5,
                 'funnel': 0.9, 'logistic_split': 0.04,
                 'linear_split': 0.02, 'logistic_wide': 0.02,
                 'logistic_d150': 0.05, 'linear_wide': 0.01,
@@ -333,15 +335,19 @@ def _nuts_one(kind, tdt, ndt, seed=0, c=64, eps=None, max_depth=10):
 
 
 @pytest.mark.parametrize('tdt,ndt,tol', DTYPES)
-@pytest.mark.parametrize('kind', ['hier', 'hier256', 'logistic', 'linear',
+@pytest.mark.parametrize('kind', ['hier', 'hier256', 'hier600', 'hier1024',
+                                  'logistic', 'linear',
                                   'diag', 'funnel', 'loggamma',
                                   'logistic_split',
                                   'linear_split', 'logistic_wide',
                                   'logistic_d150', 'linear_wide',
                                   'logistic_tall'])
 def test_nuts_one_step_matches_oracle(kind, tdt, ndt, tol):
+  # float64 states of 17 slots hold the U-turn memory of 8 levels in LDS
+  depth = 8 if kind == 'hier1024' and ndt == np.float64 else 10
   for seed in range(3):
-    spec, nxt, res, oq, ores = _nuts_one(kind, tdt, ndt, seed=seed)
+    spec, nxt, res, oq, ores = _nuts_one(kind, tdt, ndt, seed=seed,
+                                         max_depth=depth)
     np.testing.assert_array_equal(_np(res.leapfrogs_taken),
                                   ores.leapfrogs_taken)
     np.testing.assert_array_equal(_np(res.is_accepted), ores.is_accepted)
@@ -530,3 +536,26 @@ def test_zero_step_sample_chain_returns_bootstrap_results():
   np.testing.assert_array_equal(_np(inner.log_accept_ratio),
                                 _np(binner.log_accept_ratio))
   np.testing.assert_array_equal(_np(inner.energy), _np(binner.energy))
+
+
+def test_wide_float64_state_at_depth_10_is_refused():
+  # 1026 float64 elements: the U-turn memory of 10 levels (174 KiB) exceeds
+  # the workgroup's LDS; the launch is refused on the host with a message
+  o, mk, q, codes = _case('hier1024', np.float64, c=4)
+  spec = mk(torch.float64)
+  kern = pmcmc.NoUTurnSampler(
+      ptg.TransformedTarget(spec, _bijs(spec, codes)), step_size=0.1,
+      max_tree_depth=10)
+  parts = spec.unpack_state(torch.as_tensor(q, device='cuda'), (4,))
+  pkr = kern.bootstrap_results(parts)
+  with pytest.raises(RuntimeError, match='LDS'):
+    kern.one_step(parts, pkr, seed=[1, 2])
+
+
+def test_state_above_the_built_width_is_refused():
+  spec = ptg.DiagNormal(np.zeros(1100), np.ones(1100), dtype=torch.float32)
+  kern = pmcmc.HamiltonianMonteCarlo(spec, step_size=0.1,
+                                     num_leapfrog_steps=2)
+  x = torch.zeros((2, 1100), device='cuda')
+  with pytest.raises(RuntimeError, match='above 1088'):
+    kern.one_step(x, kern.bootstrap_results(x), seed=[1, 2])
