@@ -117,14 +117,14 @@ def sample_chain(num_results, current_state, previous_kernel_results=None,
   base_seed = samplers.sanitize_seed(seed, salt='mcmc.sample_chain')
   # with no step to take the step loop returns the bootstrap results as
   # they are, as the reference's loop of zero iterations does
-  if (experimental_fused and previous_kernel_results is None and
-      num_results + num_burnin_steps > 0):
+  if experimental_fused and num_results + num_burnin_steps > 0:
     plan = _fused_plan(kernel, trace_fn)
     if plan is not None:
       out = _run_fused(plan, num_results, current_state, num_burnin_steps,
                        num_steps_between_results, trace_fn,
                        return_final_kernel_results, base_seed,
-                       stats=experimental_stats)
+                       stats=experimental_stats,
+                       previous_kernel_results=previous_kernel_results)
       if out is not None:
         return out
   return _run_steps(kernel, num_results, current_state,
@@ -205,12 +205,19 @@ def _inner_results_of(plan, pkr):
 
 def _run_fused(plan, num_results, current_state, num_burnin_steps, thin,
                trace_fn, return_final, base_seed, max_iters_per_launch=4096,
-               stats=None):
+               stats=None, previous_kernel_results=None):
   """`stats` (a dict, optional) receives leapfrog totals and, if
   stats['time_kernels'] is set, per-launch HIP-event durations of the
-  sampling kernel."""
+  sampling kernel.
+
+  With `previous_kernel_results` the run resumes from them as the step loop
+  does (`mcmc/sample.py:330`): the chains start from their target log-probs
+  and gradients (and, under TTK, their transformed state), and dual
+  averaging continues from its accumulators and step counter; the
+  transitions' seeds count from 0 in each call, as in the step loop."""
   kernel = plan.outer
-  pkr = kernel.bootstrap_results(current_state)
+  pkr = (previous_kernel_results if previous_kernel_results is not None
+         else kernel.bootstrap_results(current_state))
   base_res = _inner_results_of(plan, pkr)
   spec = plan.base.target_log_prob_fn
   if plan.ttk is not None:
@@ -234,7 +241,9 @@ def _run_fused(plan, num_results, current_state, num_burnin_steps, thin,
     step0 = plan.base.step_size
   else:
     step0 = base_res.step_size
-  step = _runner.step_tensor(step0, dt, dev)
+  # a working copy: the launches update it in place, and the caller's
+  # kernel results may hold the same tensor
+  step = _runner.step_tensor(step0, dt, dev).clone()
   # traced step sizes keep the shape the step loop reports (a scalar step
   # size traces as [num_results]); list step sizes stay flat
   step_shape = (None if isinstance(step0, (list, tuple))
@@ -266,6 +275,8 @@ def _run_fused(plan, num_results, current_state, num_burnin_steps, thin,
     tlp_hist = outs['out_target_log_prob'] = torch.empty(
         (rows, c), dtype=dt, device=dev)
 
+  # dual-averaging step number of this call's first transition
+  da0 = int(pkr.step) if plan.da is not None else 0
   if plan.da is not None:
     n_adapt = int(pkr.num_adaptation_steps)
     step_shape = tuple(torch.as_tensor(step0).shape)
@@ -292,7 +303,7 @@ def _run_fused(plan, num_results, current_state, num_burnin_steps, thin,
   stream = _lib.stream_handle(dev)
   t = 0
   while t < n_total:
-    if plan.da is not None and t < n_adapt:
+    if plan.da is not None and da0 + t < n_adapt:
       n = 1
     else:
       n = min(n_total - t, max_iters_per_launch)
@@ -322,8 +333,8 @@ def _run_fused(plan, num_results, current_state, num_burnin_steps, thin,
           _lib.dtype_code(dt), gplan.n_steps, _lib.ptr(gplan.step_group),
           None,
           _lib.ptr(gplan.group_start), _lib.ptr(lar_all[t]), n, c,
-          _lib.ptr(err), _lib.ptr(avg), _lib.ptr(lst), _lib.ptr(tgt), t,
-          n_adapt, float(pkr.exploration_shrinkage),
+          _lib.ptr(err), _lib.ptr(avg), _lib.ptr(lst), _lib.ptr(tgt),
+          da0 + t, n_adapt, float(pkr.exploration_shrinkage),
           float(pkr.step_count_smoothing), float(pkr.decay_rate),
           da_lib.reduce_mode(plan.da.parameters['reduce_fn']), _lib.ptr(step),
           _lib.ptr(step64), stream))
@@ -382,7 +393,8 @@ def _run_fused(plan, num_results, current_state, num_burnin_steps, thin,
   if return_final:
     final = _final_results(plan, pkr, spec, state, tlp, grad, step, cs,
                            lar_all, outs, n_total, base_seed,
-                           (err, avg, step64) if plan.da is not None else None)
+                           (err, avg, step64) if plan.da is not None else None,
+                           da0)
     return CheckpointableStatesAndTrace(all_states, trace, final)
   if trace_fn is None:
     return all_states
@@ -419,7 +431,7 @@ def _launch_fused(plan, spec, table, state, tlp, grad, step, base_seed, n, t,
 
 
 def _final_results(plan, pkr, spec, state, tlp, grad, step, cs, lar_all, outs,
-                   n_total, base_seed, da_tensors):
+                   n_total, base_seed, da_tensors, da0=0):
   """Kernel results after the last step, in the reference's nesting."""
   last = n_total - 1
   step_val = step.reshape(torch.as_tensor(
@@ -456,6 +468,7 @@ def _final_results(plan, pkr, spec, state, tlp, grad, step, cs, lar_all, outs,
     err, avg, step64 = da_tensors
     shape = step_val.shape
     res = pkr._replace(inner_results=res, error_sum=err.reshape(shape),
-                       log_averaging_step=avg.reshape(shape), step=n_total,
+                       log_averaging_step=avg.reshape(shape),
+                       step=da0 + n_total,
                        new_step_size=step_val)
   return res

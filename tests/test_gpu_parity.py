@@ -465,6 +465,39 @@ def test_fused_sample_chain_equals_step_loop(tdt, ndt, tol):
                                 _np(b.final_kernel_results.new_step_size))
 
 
+@pytest.mark.parametrize('tdt,ndt,tol', DTYPES)
+def test_resume_in_the_fused_loop_equals_step_loop(tdt, ndt, tol):
+  # `mcmc/sample.py:330`: sample_chain(previous_kernel_results=...) resumes
+  # the chains and the adaptation; here the first call stops mid-adaptation
+  # (6 of 10 steps) and the second finishes it and samples
+  spec, da, init, _, _ = _hier_kernel(tdt, ndt)
+  tf = pmcmc.TraceFields('log_accept_ratio', 'leapfrogs_taken', 'step_size')
+  first = pmcmc.sample_chain(2, init, kernel=da, num_burnin_steps=4,
+                             trace_fn=tf, seed=5,
+                             return_final_kernel_results=True)
+  pkr = first.final_kernel_results
+  assert int(pkr.step) == 6
+  last = [p[-1] for p in first.all_states]
+  kw = dict(num_burnin_steps=3, num_steps_between_results=1, trace_fn=tf,
+            seed=8, previous_kernel_results=pkr,
+            return_final_kernel_results=True)
+  step_before = _np(pkr.new_step_size).copy()
+  a = pmcmc.sample_chain(5, last, kernel=da, **kw)
+  a2 = pmcmc.sample_chain(5, last, kernel=da, **kw)
+  b = pmcmc.sample_chain(5, last, kernel=da, experimental_fused=False, **kw)
+  # the caller's kernel results are not changed by a fused resume
+  np.testing.assert_array_equal(_np(pkr.new_step_size), step_before)
+  for pa, pa2, pb in zip(a.all_states, a2.all_states, b.all_states):
+    np.testing.assert_array_equal(_np(pa), _np(pb))
+    np.testing.assert_array_equal(_np(pa2), _np(pb))
+  for k in range(3):
+    np.testing.assert_array_equal(_np(a.trace[k]), _np(b.trace[k]))
+  fa, fb = a.final_kernel_results, b.final_kernel_results
+  assert int(fa.step) == int(fb.step) == 6 + 3 + 1 + 4 * 2
+  for f in ('new_step_size', 'error_sum', 'log_averaging_step'):
+    np.testing.assert_array_equal(_np(getattr(fa, f)), _np(getattr(fb, f)))
+
+
 def test_headline_shape_fused_equals_step_loop():
   """The headline's shape (J = 256 groups, 1024 chains, float32, NUTS + DA
   + TTK): the fused device loop equals the host step loop bit for bit."""

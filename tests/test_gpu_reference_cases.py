@@ -132,11 +132,12 @@ def _log_gamma_chains(shape, chain_ndims, seed=0):
   return torch.as_tensor(x.reshape(c, -1), device='cuda')
 
 
-@pytest.mark.parametrize('chain_ndims', [1, 2])
+@pytest.mark.parametrize('chain_ndims', [0, 1, 2])
 def test_hmc_log_gamma_expectations(chain_ndims):
   # hmc_test.py:146-200: 150 burn-in + 150 draws, step 0.05, 2 leapfrogs;
   # E[x] = digamma(5) - log(10) within 0.045, E[exp x] = 0.5 within 0.02 and
-  # every acceptance probability above 0.5.
+  # every acceptance probability above 0.5.  chain_ndims 0 is one chain of
+  # 1000 elements.
   x = _log_gamma_chains((50, 10, 2), chain_ndims)
   spec = ptg.LogGamma(x.shape[1], 5.0, 10.0)
   kern = pmcmc.HamiltonianMonteCarlo(spec, step_size=0.05,
