@@ -135,41 +135,31 @@ def ess_rhat(torch, pa, states):
 
 
 def cpu_baseline(wl, final_step, final_x, min_ess, budget_s):
-  """TFP's batched NUTS algorithm restated in NumPy (the oracle), timed on
-  one host core on a bounded number of post-adaptation transitions."""
-  from oracle import mcmc as omcmc
-  from oracle import rng as orng
-  from oracle import targets as otg
-  j = len(wl['y'])
-  codes = np.zeros(j + 2, np.int32)
-  codes[1] = 1
-  ot = otg.Transformed(otg.HierarchicalNormal(wl['y'].astype(np.float64),
-                                              wl['sigma'].astype(np.float64)),
-                       codes)
+  """TFP's batched NUTS algorithm restated in NumPy (oracle.mcmc over a
+  float32 NumPy target), timed on the host from the GPU run's final
+  (adapted) states and step size: the 1024-chain batch on one core (median
+  of 3), and the batch split over one process per core, all at once.
+  Seconds per transition x the run's transitions gives the CPU run time."""
+  import os
+  import bench_workloads as bw
+  cores = len(os.sched_getaffinity(0))
+  tg = bw._CpuHier(wl['y'], wl['sigma'])
   x = final_x.astype(np.float64)
-  t, g = ot.log_prob_and_grad(x)
-  draws = orng.PhiloxDraws(np.float32)
-  n_iter = 0
-  lf = 0
-  t0 = time.perf_counter()
-  while time.perf_counter() - t0 < budget_s:
-    x, r = omcmc.nuts_one_step(ot, final_step, x, t, g,
-                               orng.fold_in(7, n_iter), draws,
-                               max_depth=wl['max_tree_depth'])
-    t, g = r.target_log_prob, r.grads_target_log_prob
-    lf += int(r.leapfrogs_taken.sum())
-    n_iter += 1
-  el = time.perf_counter() - t0
-  per_iter = el / n_iter
-  total_iters = wl['num_adaptation_steps'] + wl['num_results']
-  return dict(value=float(min_ess / (per_iter * total_iters)),
-              unit='min-ESS/s', cores=1, kind='port',
-              sample=(f'{n_iter} post-adaptation NUTS transitions of the '
-                      f'{wl["chains"]}-chain batch ({el:.1f} s, '
-                      f'{lf / el:.0f} chain-grads/s), extrapolated to the '
-                      f'{total_iters}-transition run with the GPU run\'s '
-                      'min-ESS; NumPy float64, 1 thread'),
-              chain_grads_per_s=float(lf / el))
+  total = wl['num_adaptation_steps'] + wl['num_results']
+  budget = max(1.0, budget_s / 2.0)
+  t_one = bw._cpu_nuts_one(tg, x, final_step, budget)
+  parts = np.array_split(x, cores)
+  t_all = max(bw._cpu_nuts_all([(tg, p, final_step) for p in parts], budget))
+  return dict(value=float(min_ess / (t_all * total)), unit='min-ESS/s',
+              cores=cores, kind='port',
+              sample=(f'post-adaptation NUTS transitions of the '
+                      f'{wl["chains"]}-chain batch, float32 NumPy target; '
+                      f'{cores} processes with {len(parts[0])} chains each '
+                      f'({t_all:.2f} s per transition), extrapolated to the '
+                      f'{total}-transition run with the GPU run\'s min-ESS'),
+              seconds_per_run=t_all * total,
+              one_thread=dict(value=float(min_ess / (t_one * total)),
+                              seconds_per_run=t_one * total, cores=1))
 
 
 def main():
@@ -285,6 +275,10 @@ def main():
       'roofline': {
           'bound': 'mfma', 'achieved': achieved, 'peak': FP32_PEAK_TFLOPS,
           'unit': 'TFLOP/s', 'frac': achieved / FP32_PEAK_TFLOPS,
+          'limiter': ('VALU/latency: one wave per chain, FP32 FMA and '
+                      'cross-lane reductions, no MFMA; priced against the '
+                      'FP32 dense peak; adaptation launches are paced by '
+                      'the longest trajectory of 1024 chains'),
           'traffic': NUTS_LAUNCH_TRAFFIC_BYTES,
           'traffic_what': 'HBM bytes per adaptation launch (rocprofv3 '
                           'FETCH_SIZE/WRITE_SIZE, copy-calibrated; '

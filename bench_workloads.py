@@ -38,11 +38,22 @@ import numpy as np
 FP32_PEAK_TFLOPS = 157.3
 BF16_PEAK_TFLOPS = 2500.0
 HBM_PEAK_GBS = 8000.0
+# The per-chain sampling kernels issue no matrix instructions: one wave per
+# chain runs FP32 FMA chains, cross-lane reductions and FP64 energy sums, so
+# they sit under the compute roof, priced at the FP32 dense peak, and are
+# held far below it by instruction latency within each wave.
+VALU_LIMITER = ('VALU/latency: one wave per chain, FP32 FMA and cross-lane '
+                'reductions, no MFMA; priced against the FP32 dense peak')
 
 
-def _base(metric, value, unit, args, ms, dtype, config, higher=True):
+def _base(metric, value, unit, args, ms, dtype, config, higher=True,
+          steps=None, warmup=None):
+  """The headline fields; steps/warmup are the timed and untimed steps the
+  workload actually ran (the arguments' when not given)."""
   return {'metric': metric, 'value': value, 'unit': unit, 'n_gpus': 1,
-          'steps': args.steps, 'warmup': args.warmup, 'ms_per_step': ms,
+          'steps': args.steps if steps is None else steps,
+          'warmup': args.warmup if warmup is None else warmup,
+          'ms_per_step': ms,
           'higher_is_better': higher, 'scaling': 'weak', 'vs_baseline': None,
           'dtype': dtype, 'data': 'synthetic (seeded NumPy PCG64)',
           'config': config}
@@ -117,6 +128,7 @@ def logistic_1024x25(args, torch, pa):
       'roofline': {'bound': 'mfma', 'achieved': achieved,
                    'peak': FP32_PEAK_TFLOPS, 'unit': 'TFLOP/s',
                    'frac': achieved / FP32_PEAK_TFLOPS, 'traffic': None,
+                   'limiter': VALU_LIMITER,
                    'kernel': 'hmc_kernel<float,1,glm,calibrated> (all 1500 '
                              'transitions in one launch)',
                    'kernel_ms': kern_ms,
@@ -126,36 +138,62 @@ def logistic_1024x25(args, torch, pa):
     line['plain'] = {'value': min_ess / (pms / 1e3), 'ms_per_step': pms,
                      'what': 'host step loop, one hmc launch per transition'}
   if not args.no_cpu:
-    line['cpu_baseline'] = _cpu_hmc(wl, min_ess, args.cpu_seconds)
+    line['cpu_baseline'] = _cpu_glm_hmc(
+        wl['x'], wl['y'], c, L, wl['step_size'], nb + nr, min_ess,
+        args.cpu_seconds, 'the 64-chain batch')
   return line
 
 
-def _cpu_hmc(wl, min_ess, budget_s):
+def _cpu_glm_hmc(x, y, chains, L, eps, transitions, min_ess, budget_s,
+                 what, sample_chains=None, one_thread_chains=None):
+  """TFP's HMC restated in NumPy (oracle.mcmc.hmc_one_step) over the
+  logistic-regression log-density with float32 design-matrix products
+  (_CpuGlm), timed on the host: transitions of `sample_chains` chains
+  (default all) within a time budget, the median of 3 repeats after a
+  warm-up, with BLAS on every core of the process's affinity mask and on one
+  (`one_thread_chains` chains).  The cost scales linearly in chains (the
+  products are GEMMs over chains) to the run's `chains` x `transitions`."""
+  import os
+  from threadpoolctl import threadpool_limits
   from oracle import mcmc as omcmc
   from oracle import rng as orng
-  from oracle import targets as otg
-  ot = otg.LogisticRegression(wl['x'].astype(np.float64),
-                              wl['y'].astype(np.float64))
-  c, d = wl['chains'], wl['x'].shape[1]
-  q = np.zeros((c, d))
-  t, g = ot.log_prob_and_grad(q)
-  draws = orng.PhiloxDraws(np.float32)
-  n_iter = 0
-  t0 = time.perf_counter()
-  while time.perf_counter() - t0 < budget_s:
-    q, r = omcmc.hmc_one_step(ot, wl['step_size'], wl['num_leapfrog_steps'],
-                              q, t, g, orng.fold_in(7, n_iter), draws)
-    t, g = r.target_log_prob, r.grads_target_log_prob
-    n_iter += 1
-  el = time.perf_counter() - t0
-  total = wl['num_burnin_steps'] + wl['num_results']
-  return {'value': float(min_ess / (el / n_iter * total)), 'unit': 'min-ESS/s',
-          'cores': 1, 'kind': 'port',
-          'sample': (f'{n_iter} HMC transitions of the 64-chain batch '
-                     f'({el:.1f} s), extrapolated to {total} transitions with '
-                     'the GPU run\'s min-ESS; NumPy float64, 1 thread'),
-          'chain_grads_per_s': float(n_iter * c * wl['num_leapfrog_steps'] /
-                                     el)}
+  cores = len(os.sched_getaffinity(0))
+  tg = _CpuGlm(x, y, False)
+  rep_s = max(0.5, budget_s / 8.0)
+
+  def per_transition(threads, c):
+    with threadpool_limits(limits=threads):
+      q = np.zeros((c, x.shape[1]))
+      t, g = tg.log_prob_and_grad(q)
+      reps = []
+      for rep in range(4):   # first is warm-up
+        n, t0 = 0, time.perf_counter()
+        while time.perf_counter() - t0 < rep_s or n == 0:
+          q, r = omcmc.hmc_one_step(tg, eps, L, q, t, g, orng.fold_in(7, n),
+                                    orng.PhiloxDraws(np.float32))
+          t, g = r.target_log_prob, r.grads_target_log_prob
+          n += 1
+        if rep:
+          reps.append((time.perf_counter() - t0) / n)
+    return float(np.median(reps)) * chains / c
+
+  ca = sample_chains or chains
+  co = one_thread_chains or ca
+  s_all = per_transition(cores, ca) * transitions
+  s_one = per_transition(1, co) * transitions
+  scaled = ('' if ca == chains else
+            f' ({ca} of the {chains} chains timed, scaled by {chains / ca:g})')
+  return {'value': float(min_ess / s_all), 'unit': 'min-ESS/s',
+          'cores': cores, 'kind': 'port',
+          'sample': (f'HMC transitions of {what}{scaled}, median of 3 after '
+                     f'a warm-up, NumPy with float32 design-matrix products '
+                     f'on {cores} BLAS threads, extrapolated to '
+                     f'{transitions} transitions = {s_all:.0f} s with the GPU '
+                     'run\'s min-ESS'),
+          'seconds_per_run': s_all,
+          'one_thread': {'value': float(min_ess / s_one),
+                         'seconds_per_run': s_one, 'cores': 1,
+                         'chains_timed': co}}
 
 
 # ------------------------------------------------ C2 large-batch MFMA HMC
@@ -178,11 +216,11 @@ def logistic_65536x256(args, torch, pa):
         trace_fn=pa.mcmc.TraceFields('is_accepted'), seed=args.seed + k,
         return_final_kernel_results=True, experimental_stats=stats)
 
-  steps = max(1, min(args.steps, 1))
+  # one step is one complete run (500 + 1000 transitions of 4096 chains)
+  steps, warm = 1, min(args.warmup, 1)
   _say('mfma: runs')
   stats = {'time_kernels': True}
-  res, ms = _timed(torch, lambda k: run(k, stats=stats), steps,
-                   min(args.warmup, 1))
+  res, ms = _timed(torch, lambda k: run(k, stats=stats), steps, warm)
   _say(f'mfma: {ms:.0f} ms; diagnostics')
   draws = res.all_states
   acc = float(res.trace.float().mean())
@@ -198,6 +236,21 @@ def logistic_65536x256(args, torch, pa):
   sd = draws.double().std((0, 1))
   mcse = sd / torch.sqrt(ess.double() / 2)
   z_halves = float(((m1 - m2).abs() / (math.sqrt(2) * mcse)).max())
+  # error of the MFMA log-density and gradient against float64 on the same
+  # BF16-rounded X, at the final states of all chains
+  q = draws[-1]
+  tlp_m, grad_m = spec.log_prob_and_grad_flat(q)
+  xr, y64, q64 = spec.x.double(), spec.y.double(), q.double()
+  sc = q64 @ xr.t()
+  tlp_r = ((y64 * sc - torch.nn.functional.softplus(sc)).sum(1)
+           - 0.5 * (q64 ** 2).sum(1) - d * 0.5 * math.log(2 * math.pi))
+  grad_r = (y64 - torch.sigmoid(sc)) @ xr - q64
+  del sc
+  nw = lambda a, b: float((a.double() - b).abs().max() / b.abs().max())
+  err64 = {'target_log_prob': nw(tlp_m, tlp_r), 'grad': nw(grad_m, grad_r),
+           'what': 'normwise max|err| / max|value| of the MFMA path vs '
+                   'float64 on the BF16-rounded X at the final states of '
+                   'all 4096 chains'}
   # kernel time of the gradient GEMMs: HIP events per launch
   launches = stats['launches']
   sample_ms = sum(l[0] for l in launches if l[2] > 1)
@@ -210,13 +263,16 @@ def logistic_65536x256(args, torch, pa):
                'min-ESS/s', args, ms, 'bf16 x, f32 state',
                {'workload': 'logistic_65536x256: DualAveraging(HMC(L=20), '
                             'target 0.75, 500 steps) + 1000 results, X '
-                            'N(0,1)/16 in BF16, 4096 chains at 0',
-                'chains': c, 'n_obs': n, 'dim': d})
+                            'N(0,1)/16 in BF16, 4096 chains at 0; one step '
+                            '= one complete run',
+                'chains': c, 'n_obs': n, 'dim': d},
+               steps=steps, warmup=warm)
   line.update({
       'min_ess': min_ess, 'rhat_max': float(rhat.max()),
       'rhat_ok': float(rhat.max()) < 1.01, 'accept_rate': acc,
       'final_step_size': float(res.final_kernel_results.new_step_size),
       'mean_halves_max_z': z_halves,
+      'error_vs_fp64': err64,
       'chain_grad_evals_per_s': c * (nb + nr) * L / (ms / 1e3),
       'roofline': {'bound': 'mfma', 'achieved': achieved,
                    'peak': BF16_PEAK_TFLOPS, 'unit': 'TFLOP/s',
@@ -226,6 +282,11 @@ def logistic_65536x256(args, torch, pa):
                    'algorithmic_flop_per_gradient': flop_grad,
                    'executed_mfma_flop_per_gradient': 2 * flop_grad,
                    'sampling_ms': sample_ms}})
+  if not args.no_cpu:
+    line['cpu_baseline'] = _cpu_glm_hmc(
+        x, y, c, L, float(res.final_kernel_results.new_step_size), nb + nr,
+        min_ess, args.cpu_seconds, 'the 65536 x 256 problem',
+        sample_chains=256, one_thread_chains=32)
   return line
 
 
@@ -454,6 +515,86 @@ class _CpuGlm:
     return lp, (r @ self.x).astype(np.float64) - q
 
 
+class _CpuHier:
+  """The non-centred hierarchical normal with Softplus on tau, in float32
+  NumPy over unconstrained [mu, tau_u, z] (the CPU restatement's target;
+  oracle.targets holds the float64 one): mu ~ N(0, 5), tau ~ HalfCauchy(0,
+  5), z ~ N(0, 1), y ~ N(mu + tau z, sigma), plus log sigmoid(tau_u), the
+  Softplus log-Jacobian."""
+
+  def __init__(self, y, sigma, mu_scale=5.0, tau_scale=5.0):
+    self.y = np.asarray(y, np.float32)
+    self.s2 = np.asarray(sigma, np.float32) ** 2
+    j = len(self.y)
+    # normalising constants: the likelihood's and z's Gaussians, mu's
+    # Gaussian, tau's half-Cauchy
+    self.c = np.float32(-np.log(np.asarray(sigma, np.float64)).sum()
+                        - j * np.log(2 * np.pi)
+                        - 0.5 * np.log(2 * np.pi) - np.log(mu_scale)
+                        + np.log(2.0 / (np.pi * tau_scale)))
+    self.ms, self.ts = np.float32(mu_scale), np.float32(tau_scale)
+
+  def log_prob_and_grad(self, q):
+    q = np.asarray(q, np.float32)
+    mu, tu, z = q[:, 0], q[:, 1], q[:, 2:]
+    tau = np.logaddexp(np.float32(0), tu).astype(np.float32)
+    sg = (1 / (1 + np.exp(-tu))).astype(np.float32)
+    r = (self.y - (mu[:, None] + tau[:, None] * z)) / self.s2
+    lp = (-0.5 * (mu / self.ms) ** 2
+          - np.log1p((tau / self.ts) ** 2) + np.log(sg)
+          - 0.5 * (z * z).sum(1)
+          - 0.5 * (r * r * self.s2).sum(1) + self.c)
+    g_tau = -2 * tau / (self.ts ** 2 + tau * tau) + (r * z).sum(1)
+    g = np.empty_like(q)
+    g[:, 0] = -mu / self.ms ** 2 + r.sum(1)
+    g[:, 1] = g_tau * sg + (1 - sg)
+    g[:, 2:] = -z + tau[:, None] * r
+    return lp.astype(np.float64), g.astype(np.float64)
+
+
+def _cpu_nuts_worker(target, q, eps, budget_s, seed):
+  """NUTS transitions (oracle.mcmc.nuts_one_step) of the chains q on one
+  core for budget_s seconds after one warm-up transition; returns
+  (transitions, seconds, chain-leapfrogs)."""
+  from threadpoolctl import threadpool_limits
+  from oracle import mcmc as omcmc
+  from oracle import rng as orng
+  with threadpool_limits(limits=1):
+    t, g = target.log_prob_and_grad(q)
+    n, lf, t0 = 0, 0, None
+    while t0 is None or time.perf_counter() - t0 < budget_s:
+      q, r = omcmc.nuts_one_step(target, eps, q, t, g, orng.fold_in(seed, n),
+                                 orng.PhiloxDraws(np.float32), max_depth=10)
+      t, g = r.target_log_prob, r.grads_target_log_prob
+      if t0 is None:
+        t0 = time.perf_counter()
+      else:
+        n += 1
+        lf += int(r.leapfrogs_taken.sum())
+    return n, time.perf_counter() - t0, lf
+
+
+def _cpu_nuts_one(target, q, eps, budget_s):
+  """Seconds per NUTS transition of the chains q on one core: the median of
+  3 timed stretches of budget_s / 3 seconds."""
+  reps = [_cpu_nuts_worker(target, q, eps, budget_s / 3, 5) for _ in range(3)]
+  return float(np.median([s / max(n, 1) for n, s, _ in reps]))
+
+
+def _cpu_nuts_all(jobs, budget_s):
+  """The jobs (target, chains q, step size), one process per job, all at
+  once (spawned children, one per core of the affinity mask); seconds per
+  transition of each job under that load."""
+  import concurrent.futures
+  import multiprocessing
+  ctx = multiprocessing.get_context('spawn')
+  with concurrent.futures.ProcessPoolExecutor(len(jobs),
+                                              mp_context=ctx) as ex:
+    futs = [ex.submit(_cpu_nuts_worker, tg, q, eps, budget_s, 5 + k)
+            for k, (tg, q, eps) in enumerate(jobs)]
+    return [s / max(n, 1) for n, s, _ in (f.result() for f in futs)]
+
+
 def _ragged_cpu(args, specs, line, ess, ok):
   """TFP's batched NUTS restated in NumPy (oracle.mcmc.nuts_one_step over
   the float32 GLM above), timed on the host: three problems (smallest,
@@ -568,31 +709,82 @@ def serving_poisson200(args, torch, pa):
       req.draws = None
     if time.perf_counter() - last_print > 20:
       last_print = time.perf_counter()
-      print(f'serving: {len(lat)}/{n_req} done, {len(srv.active)} active, '
-            f'{len(srv.queue)} queued, t={last_print - t0:.0f}s', flush=True)
+      _say(f'serving: {len(lat)}/{n_req} done, {len(srv.active)} active, '
+           f'{len(srv.queue)} queued, t={last_print - t0:.0f}s')
   wall = time.perf_counter() - t0
   lat = np.array(lat)
   ess_min = np.array(ess_min)
   rhat_max = np.array(rhat_max)
+  ok = rhat_max < 1.01
   line = _base('served posterior-sampling requests: throughput with p50/p99 '
                'latency (Poisson 200 req/s, NUTS+DA, 16 chains)',
                n_req / wall, 'req/s', args, 1000.0 * wall, 'f32',
                {'workload': f'serving_poisson200: {n_req} requests, half '
                             'ragged GLMs (N~LogU[128,8192], D~U{4..128}), '
                             'half hierarchical normals (J~U{8..1024}, Softplus '
-                            'on tau); 16 chains, 500 adaptation + 2000 draws',
+                            'on tau); 16 chains, 500 adaptation + 2000 draws; '
+                            'one step = the whole arrival process',
                 'requests': n_req, 'arrival_rate': rate,
-                'hbm_budget_gib': args.hbm_budget_gib})
+                'hbm_budget_gib': args.hbm_budget_gib},
+               steps=1, warmup=0)
   line.update({
       'latency_s_p50_p99': [float(np.percentile(lat, 50)),
                             float(np.percentile(lat, 99))],
       'min_ess_per_request_min_median': [float(ess_min.min()),
                                          float(np.median(ess_min))],
-      'aggregate_min_ess_per_s': float(ess_min.sum()) / wall,
-      'rhat_ok_fraction': float((rhat_max < 1.01).mean()),
+      'aggregate_min_ess_per_s': float(ess_min[ok].sum()) / wall,
+      'aggregate_counts': 'requests at split-Rhat < 1.01 only',
+      'requests_failing_rhat': int((~ok).sum()),
+      'rhat_ok_fraction': float(ok.mean()),
       'rhat_max_median': float(np.median(rhat_max)),
       'ticks': srv.ticks})
+  if not args.no_cpu:
+    from probability_amd import _lib
+    glm = next(r for r in reqs if r.spec.kind != _lib.TARGET_HIER_NORMAL)
+    hier = next(r for r in reqs if r.spec.kind == _lib.TARGET_HIER_NORMAL)
+    line['cpu_baseline'] = _serving_cpu(args, [glm, hier], 2500)
   return line
+
+
+def _serving_cpu(args, samples, transitions):
+  """Requests served by TFP's NUTS restated in NumPy (oracle.mcmc over the
+  float32 targets above): post-adaptation NUTS transitions of 16 chains of a
+  sampled GLM and hierarchical request at their adapted step sizes, on one
+  core (median of 3) and with every core busy (one process per core, half
+  on each request).  A request costs its seconds per transition x
+  `transitions` of one core; the cores serve the half-and-half mix at
+  cores / (mean cost) requests per second."""
+  import os
+  from probability_amd import _lib
+  cores = len(os.sched_getaffinity(0))
+  jobs, what = [], []
+  for r in samples:
+    spec = r.spec
+    if spec.kind == _lib.TARGET_HIER_NORMAL:
+      tg = _CpuHier(spec.y.cpu().numpy(), spec.sigma.cpu().numpy())
+    else:
+      lin = spec.kind == _lib.TARGET_LINEAR
+      tg = _CpuGlm(spec.x.cpu().numpy()[:, :spec.dim - (1 if lin else 0)],
+                   spec.y.cpu().numpy(), lin)
+    jobs.append((tg, np.zeros((r.num_chains, spec.dim)),
+                 float(r.final_step_size)))
+    what.append(f'kind {spec.kind} dim {spec.dim}')
+  budget = max(1.0, args.cpu_seconds / 4.0)
+  one = [_cpu_nuts_one(tg, q, eps, budget / 2) for tg, q, eps in jobs]
+  loaded = _cpu_nuts_all([jobs[k % len(jobs)] for k in range(cores)], budget)
+  per_kind = [float(np.mean(loaded[k::len(jobs)])) for k in range(len(jobs))]
+  s_one = float(np.mean(one)) * transitions
+  s_all = float(np.mean(per_kind)) * transitions
+  return {'value': cores / s_all, 'unit': 'req/s', 'cores': cores,
+          'kind': 'port',
+          'sample': ('NUTS transitions of a GLM and a hierarchical request '
+                     '(' + ', '.join(what) + '; 16 chains at their adapted '
+                     'step sizes), float32 NumPy targets; one core (median '
+                     'of 3) and one process per core; x ' + str(transitions) +
+                     ' transitions per request'),
+          'core_seconds_per_request_loaded': s_all,
+          'one_thread': {'value': 1.0 / s_one, 'cores': 1,
+                         'seconds_per_request': s_one}}
 
 
 # ------------------------------------------------------------ reducers
