@@ -316,13 +316,19 @@ C3_STEP_TRANSITIONS = 75     # one bench step: 1/20 of a 1500-transition run
 C3_HBM_BYTES_PER_ENTRY = (1068320259 * 2.0 + 230789418 * 1.0) * 1024 / 4.878e12
 
 
+# sampling launches of the 8- and 4-wave classes run chain groups of 4
+# (one pass over X per gradient for 4 chains of a problem)
+C3_CHAINS_PER_BLOCK = {8: 4, 4: 4}
+
+
 def _ragged_run(pa, specs, bij, seed, adaptation, na, nr, tap):
   from probability_amd.mcmc import batched
   return batched.RaggedRun(
       specs, 32, nr, bijectors=bij, slot_dim=192, seed=seed,
       num_burnin_steps=na, num_adaptation_steps=na, step_size=0.05,
       target_accept_prob=tap, max_tree_depth=10, max_iters_per_launch=100,
-      adaptation=adaptation, stats={'time_kernels': False})
+      adaptation=adaptation, stats={'time_kernels': False},
+      chains_per_block=C3_CHAINS_PER_BLOCK)
 
 
 def _ragged_gate(torch, pa, draws):
@@ -399,7 +405,8 @@ def ragged_glms_512(args, torch, pa):
                             num_burnin_steps=na, num_adaptation_steps=na,
                             step_size=0.05, target_accept_prob=tap,
                             max_tree_depth=10, max_iters_per_launch=100,
-                            adaptation=adaptation)
+                            adaptation=adaptation,
+                            chains_per_block=C3_CHAINS_PER_BLOCK)
     torch.cuda.synchronize()
     ta = time.perf_counter()
     one.advance_to(total)

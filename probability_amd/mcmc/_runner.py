@@ -8,6 +8,7 @@ import numpy as np
 import torch
 
 from probability_amd import _lib
+from probability_amd import tuning
 
 _OUT_FIELDS = ('out_log_accept_ratio', 'out_leapfrogs', 'out_is_accepted',
                'out_reach_max_depth', 'out_has_divergence', 'out_energy',
@@ -34,7 +35,7 @@ class Run:
   """Owns the output buffers of one kernel launch."""
 
   def __init__(self, spec, table, state, tlp, grad, step, seed, n_iters=1,
-               fold_seed=False, iter0=0):
+               fold_seed=False, iter0=0, sampler='nuts'):
     _lib.require_cuda()
     self.spec = spec
     self.table = table
@@ -55,7 +56,9 @@ class Run:
     a.target_kind = spec.kind
     if spec.kind in (_lib.TARGET_LOGISTIC, _lib.TARGET_LINEAR):
       base = getattr(spec, '_base', spec)
-      a.waves_per_chain = _lib.glm_waves_per_chain(base.n_obs * base.dim)
+      # sampler: 'nuts' or 'hmc' (the HMC and MALA kernels)
+      a.waves_per_chain = tuning.glm_waves(spec.kind, base.n_obs, base.dim,
+                                           c, sampler, state.dtype)
     a.max_energy_diff = 1000.0
     a.problems = table.data_ptr()
     a.n_problems = 1

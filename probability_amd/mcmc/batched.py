@@ -20,6 +20,7 @@ import numpy as np
 import torch
 
 from probability_amd import _lib
+from probability_amd import tuning
 from probability_amd import bijectors as bij_lib
 from probability_amd import random as samplers
 from probability_amd import targets as targets_lib
@@ -115,8 +116,10 @@ class PosteriorBatch:
     elif waves_per_chain is not None:
       self.waves_per_chain = int(waves_per_chain)
     else:
-      self.waves_per_chain = _lib.glm_waves_per_chain(
-          max(s.n_obs * s.dim for s in self.specs))
+      big = max(self.specs, key=lambda s: s.n_obs * s.dim)
+      self.waves_per_chain = tuning.glm_waves(
+          big.kind, big.n_obs, big.dim, max(self.num_chains), 'nuts',
+          big.dtype)
 
   @property
   def n_problems(self):
@@ -229,8 +232,12 @@ class RaggedRun:
     bijs = bijectors or [None] * p
     keys = list(range(p)) if keys is None else [int(k) for k in keys]
     glm = _family(specs[0].kind) == 1
-    cls = [_lib.glm_waves_per_chain(s.n_obs * s.dim) if glm else 1
-           for s in specs]
+    # a problem's class depends on its own size and chain count only (the
+    # tuned wave count for its chains alone), so its draws equal its draws
+    # alone
+    cls = [tuning.glm_waves(s.kind, s.n_obs, s.dim, chains[i], 'nuts',
+                            s.dtype) if glm else 1
+           for i, s in enumerate(specs)]
     dev = specs[0].device
     self.caller = torch.cuda.current_stream(dev)
     self.runs = []

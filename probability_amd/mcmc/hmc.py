@@ -97,7 +97,8 @@ class UncalibratedHamiltonianMonteCarlo(kernel_base.TransitionKernel):
     seed = samplers.sanitize_seed(seed)
     run = _runner.Run(spec, spec.table(spec.bijectors_for_eval()), state, tlp,
                       grad, _runner.step_tensor(step_size, spec.dtype,
-                                                spec.device), seed)
+                                                spec.device), seed,
+                      sampler='hmc')
     run.args.num_leapfrog_steps = nsteps
     c, d = state.shape
     corr = run.want('out_energy', (1, c), spec.dtype)
@@ -198,7 +199,8 @@ class HamiltonianMonteCarlo(kernel_base.TransitionKernel):
     seed = samplers.sanitize_seed(seed)
     run = _runner.Run(spec, spec.table(spec.bijectors_for_eval()), state, tlp,
                       grad, _runner.step_tensor(step_size, spec.dtype,
-                                                spec.device), seed)
+                                                spec.device), seed,
+                      sampler='hmc')
     run.args.num_leapfrog_steps = nsteps
     c, d = state.shape
     lar = run.want('out_log_accept_ratio', (1, c), spec.dtype)

@@ -107,7 +107,7 @@ class UncalibratedLangevin(kernel_base.TransitionKernel):
     h = _leapfrog_step(self.step_size, spec.dtype, spec.device)
     run = _runner.Run(spec, spec.table(spec.bijectors_for_eval()), state, tlp,
                       grad, _runner.step_tensor(h, spec.dtype, spec.device),
-                      seed)
+                      seed, sampler='hmc')
     run.args.num_leapfrog_steps = 1
     c, _ = state.shape
     corr = run.want('out_energy', (1, c), spec.dtype)
@@ -171,7 +171,7 @@ class MetropolisAdjustedLangevinAlgorithm(kernel_base.TransitionKernel):
     h = _leapfrog_step(self.step_size, spec.dtype, spec.device)
     run = _runner.Run(spec, spec.table(spec.bijectors_for_eval()), state, tlp,
                       grad, _runner.step_tensor(h, spec.dtype, spec.device),
-                      seed)
+                      seed, sampler='hmc')
     run.args.num_leapfrog_steps = 1
     c, d = state.shape
     lar = run.want('out_log_accept_ratio', (1, c), spec.dtype)

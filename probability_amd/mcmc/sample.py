@@ -406,7 +406,8 @@ def _launch_fused(plan, spec, table, state, tlp, grad, step, base_seed, n, t,
                   num_results, fn, stream, events):
   """One sampler launch of the fused loop over iterations [t, t + n)."""
   run = _runner.Run(spec, table, state, tlp, grad, step, base_seed,
-                    n_iters=n, fold_seed=True, iter0=t)
+                    n_iters=n, fold_seed=True, iter0=t,
+                    sampler='nuts' if plan.kind == 'nuts' else 'hmc')
   a = run.args
   if plan.kind == 'nuts':
     plan.base._configure(run)
