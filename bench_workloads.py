@@ -321,11 +321,21 @@ C3_HBM_BYTES_PER_ENTRY = (1068320259 * 2.0 + 230789418 * 1.0) * 1024 / 4.878e12
 C3_CHAINS_PER_BLOCK = {8: 4, 4: 4}
 
 
+def _init_steps(specs, adaptation):
+  """Initial step sizes: windowed adaptation starts each problem at the
+  reference's windowed default 0.5 D^-1/4 (`_get_step_size`); dual
+  averaging alone at 0.05."""
+  if adaptation == 'windowed':
+    return [0.5 * float(s.dim) ** -0.25 for s in specs]
+  return 0.05
+
+
 def _ragged_run(pa, specs, bij, seed, adaptation, na, nr, tap):
   from probability_amd.mcmc import batched
   return batched.RaggedRun(
       specs, 32, nr, bijectors=bij, slot_dim=192, seed=seed,
-      num_burnin_steps=na, num_adaptation_steps=na, step_size=0.05,
+      num_burnin_steps=na, num_adaptation_steps=na,
+      step_size=_init_steps(specs, adaptation),
       target_accept_prob=tap, max_tree_depth=10, max_iters_per_launch=100,
       adaptation=adaptation, stats={'time_kernels': False},
       chains_per_block=C3_CHAINS_PER_BLOCK)
@@ -395,18 +405,20 @@ def ragged_glms_512(args, torch, pa):
   counted = float(sum(float(e[r < 1.01].sum()) for e, r in gates))
   value = counted / wall
   # one posterior per launch: a stratified sample of problems, each alone
-  # with the same code, keys, size class and adaptation (its draws equal
-  # its draws in the batch)
+  # with the same code, keys, size class and adaptation, one chain per
+  # block (chain groups only pay when many problems share the launches; a
+  # problem alone is faster without them, and its draws then differ from
+  # the batch's only by the groups' summation order)
   idx = np.linspace(0, p - 1, max(1, args.plain_sample)).round().astype(int)
   plain_s = []
   for i in idx:
     one = batched.RaggedRun([specs[i]], chains, nr, bijectors=[bij[i]],
                             keys=[int(i)], slot_dim=192, seed=args.seed,
                             num_burnin_steps=na, num_adaptation_steps=na,
-                            step_size=0.05, target_accept_prob=tap,
+                            step_size=_init_steps([specs[i]], adaptation),
+                            target_accept_prob=tap,
                             max_tree_depth=10, max_iters_per_launch=100,
-                            adaptation=adaptation,
-                            chains_per_block=C3_CHAINS_PER_BLOCK)
+                            adaptation=adaptation)
     torch.cuda.synchronize()
     ta = time.perf_counter()
     one.advance_to(total)

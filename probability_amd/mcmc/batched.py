@@ -257,8 +257,13 @@ class RaggedRun:
       sub = {} if stats is not None else None
       if sub is not None and stats.get('time_kernels'):
         sub['time_kernels'] = True
+      kw = dict(kwargs)
+      # per-problem step sizes follow their problems into the classes
+      st = kw.get('step_size')
+      if st is not None and np.ndim(st) == 1 and len(st) == p:
+        kw['step_size'] = [st[i] for i in idx]
       self.runs.append((idx, sub, _Run(batch, stream, num_results, stats=sub,
-                                       **kwargs)))
+                                       **kw)))
     self.n_total = sum(r.n_total for _, _, r in self.runs)
     if windowed_init:
       for _, _, r in self.runs:

@@ -103,20 +103,26 @@ def _ragged_problems(dtype):
 
 @pytest.mark.parametrize('dtype', [torch.float32, torch.float64])
 def test_sample_ragged_equals_each_problem_alone(dtype):
-  from probability_amd import _lib
+  from probability_amd import tuning
   specs, bij = _ragged_problems(dtype)
   chains = [4, 6, 5, 3, 7, 3]
+  # per-problem initial step sizes follow their problems into the classes
+  steps = [0.05, 0.04, 0.06, 0.05, 0.03, 0.05]
   kw = dict(num_results=4, num_burnin_steps=5, num_adaptation_steps=5,
-            step_size=0.05, target_accept_prob=0.8, seed=9)
+            target_accept_prob=0.8, seed=9)
   stats = {}
   mix = batched.sample_ragged(specs, chains, bijectors=bij, slot_dim=64,
-                              stats=stats, **kw)
-  assert sorted(w for _, w in stats['classes']) == [1, 2, 4, 8]
+                              stats=stats, step_size=steps, **kw)
+  # each problem's class: its tuned waves per chain at its own chain count
+  waves = [tuning.glm_waves(s.kind, s.n_obs, s.dim, chains[k], 'nuts', dtype)
+           for k, s in enumerate(specs)]
+  assert sorted(w for _, w in stats['classes']) == sorted(set(waves))
+  assert len(set(waves)) > 1
   for k, s in enumerate(specs):
-    w = _lib.glm_waves_per_chain(s.n_obs * s.dim)
     alone = batched.sample_posteriors(
         batched.PosteriorBatch([s], [chains[k]], [bij[k]], keys=[k],
-                               slot_dim=64, waves_per_chain=w), **kw)
+                               slot_dim=64, waves_per_chain=waves[k]),
+        step_size=steps[k], **kw)
     np.testing.assert_array_equal(mix.draws[k].cpu().numpy(),
                                   alone.draws[0].cpu().numpy())
     np.testing.assert_array_equal(mix.leapfrogs_taken[k].cpu().numpy(),
