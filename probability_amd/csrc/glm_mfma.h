@@ -27,7 +27,7 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kMD = 256;       // state dimension of this path
 constexpr int kMRows = 32;     // rows per LDS tile
-constexpr int kMChains = 64;   // chains per workgroup (16 per wave)
+constexpr int kMChains = 64;   // chain granularity of the path (16 per wave)
 constexpr int kRPitch = kMRows + 8;    // bf16 per R row (one chain) in LDS
 
 __device__ inline f32x4 mfma16(const bf16x8& a, const bf16x8& b,
@@ -63,31 +63,47 @@ __device__ inline void lds_dma4(const void* g, void* lds_base) {
 __device__ inline int xt_swz(int d) { return ((d >> 1) ^ (d >> 2)) & 3; }
 
 // X tile: 16-byte chunk (row, c) at chunk row * 32 + (c ^ (row & 15));
-// X^T tile: chunk (d, c) at d * 4 + (c ^ xt_swz(d)).
+// X^T tile: chunk (d, c) at d * 4 + (c ^ xt_swz(d)).  The WAVES waves of the
+// workgroup share the tile's 1024 + 1024 chunk loads and its 32 labels.
+template <int WAVES>
 __device__ inline void stage_tile(const __bf16* X, const __bf16* Xt,
                                   const float* y, int n_obs, int n0,
                                   __bf16* xs, __bf16* xts, float* ys, int w,
                                   int l) {
+  constexpr int U = 16 / WAVES;      // 1 KiB chunk rows per wave and matrix
+  constexpr int YL = kMRows / WAVES; // labels per wave
   w = __builtin_amdgcn_readfirstlane(w);
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int q = (w * 4 + u) * 64 + l;
+  for (int u = 0; u < U; ++u) {
+    const int q = (w * U + u) * 64 + l;
     const int row = q >> 5, c = (q & 31) ^ (row & 15);
     lds_dma16(X + (size_t)(n0 + row) * kMD + c * 8,
-              (char*)xs + (size_t)(w * 4 + u) * 1024);
+              (char*)xs + (size_t)(w * U + u) * 1024);
   }
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int q = (w * 4 + u) * 64 + l;
+  for (int u = 0; u < U; ++u) {
+    const int q = (w * U + u) * 64 + l;
     const int d = q >> 2, c = (q & 3) ^ xt_swz(d);
     lds_dma16(Xt + (size_t)d * n_obs + n0 + c * 8,
-              (char*)xts + (size_t)(w * 4 + u) * 1024);
+              (char*)xts + (size_t)(w * U + u) * 1024);
   }
-  if (l < 8) lds_dma4(y + n0 + w * 8 + l, ys + w * 8);
+  if (l < YL) lds_dma4(y + n0 + w * YL + l, ys + w * YL);
+}
+
+// s_waitcnt immediate that waits until at most n vector-memory loads of
+// this wave are outstanding (expcnt and lgkmcnt left at their maxima).
+template <int N>
+constexpr int vmcnt_imm() {
+  static_assert(N >= 0 && N < 16, "vmcnt field");
+  return 0xF70 | N;
 }
 
 // Partial gradient (and optionally partial Bernoulli log-likelihood) of the
-// chains [64*blockIdx.x, +64) over rows [blockIdx.y*rows_per_split, +rows).
+// chains [16*WAVES*blockIdx.x, +16*WAVES) over rows
+// [blockIdx.y*rows_per_split, +rows), 16 chains per wave.  With 8 waves
+// (128 chains) every staged tile serves twice the chains of the 4-wave
+// layout, halving the X and X^T bytes staged per gradient; the workgroup
+// then takes one CU (84 KiB of LDS) with the same 8 waves per CU.
 // Tiles of 32 rows of X and X^T go straight from memory to LDS (no
 // registers) into one of two buffers while the previous tile is used, so a
 // tile's load latency hides behind the previous tile's MFMAs.  LDS layouts
@@ -96,7 +112,8 @@ __device__ inline void stage_tile(const __bf16* X, const __bf16* Xt,
 // X^T tile chunk (d, c) at d * 4 + (c ^ xt_swz(d)).  Each lane writes its
 // four consecutive residual rows as one 8-byte store (R pitch 40 bf16 keeps
 // those stores and the 16-byte fragment reads free of bank conflicts).
-__global__ void __launch_bounds__(256, 2)
+template <int WAVES>
+__global__ void __launch_bounds__(64 * WAVES, 8 / WAVES)
     glm_grad_mfma(const __bf16* __restrict__ X, const __bf16* __restrict__ Xt,
                   const float* __restrict__ y, int n_obs,
                   const __bf16* __restrict__ Bhi,
@@ -107,12 +124,13 @@ __global__ void __launch_bounds__(256, 2)
   __shared__ __align__(1024) __bf16 xs1[kMRows * kMD];
   __shared__ __align__(1024) __bf16 xts0[kMD * kMRows];
   __shared__ __align__(1024) __bf16 xts1[kMD * kMRows];
-  __shared__ __align__(16) __bf16 rs[2 * kMChains * kRPitch];
+  constexpr int CH = 16 * WAVES;  // chains of the workgroup
+  __shared__ __align__(16) __bf16 rs[2 * CH * kRPitch];
   __shared__ __align__(16) float ys0[kMRows];
   __shared__ __align__(16) float ys1[kMRows];
   const int tid = threadIdx.x;
   const int w = tid >> 6, l = tid & 63, m = l & 15, g = l >> 4;
-  const int c0 = blockIdx.x * kMChains + 16 * w;
+  const int c0 = blockIdx.x * CH + 16 * w;
   bf16x8 bh[kMD / 32], bl[kMD / 32];
 #pragma unroll
   for (int ks = 0; ks < kMD / 32; ++ks) {
@@ -132,7 +150,7 @@ __global__ void __launch_bounds__(256, 2)
   const int r0 = blockIdx.y * rows_per_split;
   const int r1 = min(n_obs, r0 + rows_per_split);
   __bf16* rh_w = rs + (16 * w) * kRPitch;
-  __bf16* rl_w = rs + kMChains * kRPitch + (16 * w) * kRPitch;
+  __bf16* rl_w = rs + CH * kRPitch + (16 * w) * kRPitch;
 
   auto compute = [&](const __bf16* xs, const __bf16* xts, const float* ys) {
     f32x4 s0 = f32x4{0.f, 0.f, 0.f, 0.f}, s1 = s0;
@@ -192,17 +210,18 @@ __global__ void __launch_bounds__(256, 2)
     }
   };
   // tiles of this split, two per round (buffers 0 and 1): the next tile's
-  // nine loads per wave are issued, then the wave waits until only those
-  // are outstanding (vmcnt(9)), i.e. until its loads of the current tile
-  // have landed; the barrier makes every wave's part visible
-  if (r0 < r1) stage_tile(X, Xt, y, n_obs, r0, xs0, xts0, ys0, w, l);
+  // LPW loads per wave are issued, then the wave waits until only those are
+  // outstanding (vmcnt(LPW)), i.e. until its loads of the current tile have
+  // landed; the barrier makes every wave's part visible
+  constexpr int LPW = 2 * (16 / WAVES) + 1;
+  if (r0 < r1) stage_tile<WAVES>(X, Xt, y, n_obs, r0, xs0, xts0, ys0, w, l);
   for (int n0 = r0; n0 < r1; n0 += 2 * kMRows) {
     const int n1 = n0 + kMRows;
     if (n1 < r1) {
-      stage_tile(X, Xt, y, n_obs, n1, xs1, xts1, ys1, w, l);
-      __builtin_amdgcn_s_waitcnt(0xF79);  // vmcnt(9): tile n0 has landed
+      stage_tile<WAVES>(X, Xt, y, n_obs, n1, xs1, xts1, ys1, w, l);
+      __builtin_amdgcn_s_waitcnt(vmcnt_imm<LPW>());  // tile n0 has landed
     } else {
-      __builtin_amdgcn_s_waitcnt(0xF70);  // vmcnt(0)
+      __builtin_amdgcn_s_waitcnt(vmcnt_imm<0>());
     }
     __syncthreads();
     compute(xs0, xts0, ys0);
@@ -210,10 +229,10 @@ __global__ void __launch_bounds__(256, 2)
     if (n1 >= r1) break;
     const int n2 = n1 + kMRows;
     if (n2 < r1) {
-      stage_tile(X, Xt, y, n_obs, n2, xs0, xts0, ys0, w, l);
-      __builtin_amdgcn_s_waitcnt(0xF79);
+      stage_tile<WAVES>(X, Xt, y, n_obs, n2, xs0, xts0, ys0, w, l);
+      __builtin_amdgcn_s_waitcnt(vmcnt_imm<LPW>());
     } else {
-      __builtin_amdgcn_s_waitcnt(0xF70);
+      __builtin_amdgcn_s_waitcnt(vmcnt_imm<0>());
     }
     __syncthreads();
     compute(xs1, xts1, ys1);

@@ -75,14 +75,25 @@ int launch_glm_hmc(const hipmc_glm_hmc_args* a, hipStream_t s) {
   const int n_split = (a->n_obs + rps - 1) / rps;
   if (n_split != a->n_split)
     return fail(HIPMC_E_INVALID, "glm_hmc: n_split leaves an empty split; use fewer");
-  const dim3 ggrid(a->n_chains / kMChains, n_split);
+  // 128 chains per workgroup when the chain count allows (each staged tile
+  // then serves 8 waves), else 64
+  const bool wide = a->n_chains % (2 * kMChains) == 0;
+  const dim3 ggrid(a->n_chains / (wide ? 2 * kMChains : kMChains), n_split);
   const dim3 cgrid((a->n_chains + 3) / 4);
   auto grad = [&](bool ll) {
-    hipLaunchKernelGGL(glm_grad_mfma, ggrid, dim3(256), 0, s,
-                       (const __bf16*)a->x_bf16, (const __bf16*)a->xt_bf16,
-                       a->y, a->n_obs, (const __bf16*)a->ws_b,
-                       (const __bf16*)a->ws_b + (size_t)a->n_chains * kMD,
-                       a->n_chains, rps, a->ws_g, ll ? a->ws_ll : nullptr);
+    const __bf16* bh = (const __bf16*)a->ws_b;
+    const __bf16* bl = bh + (size_t)a->n_chains * kMD;
+    float* oll = ll ? a->ws_ll : nullptr;
+    if (wide)
+      hipLaunchKernelGGL(glm_grad_mfma<8>, ggrid, dim3(512), 0, s,
+                         (const __bf16*)a->x_bf16, (const __bf16*)a->xt_bf16,
+                         a->y, a->n_obs, bh, bl, a->n_chains, rps, a->ws_g,
+                         oll);
+    else
+      hipLaunchKernelGGL(glm_grad_mfma<4>, ggrid, dim3(256), 0, s,
+                         (const __bf16*)a->x_bf16, (const __bf16*)a->xt_bf16,
+                         a->y, a->n_obs, bh, bl, a->n_chains, rps, a->ws_g,
+                         oll);
   };
   if (a->want_grad_only) {
     const size_t ne = (size_t)a->n_chains * kMD;
