@@ -197,6 +197,11 @@ def _cpu_glm_hmc(x, y, chains, L, eps, transitions, min_ess, budget_s,
 
 
 # ------------------------------------------------ C2 large-batch MFMA HMC
+# HBM bytes per all-chain gradient of glm_grad_mfma<8> at 65536 x 256 x 4096:
+# rocprofv3 FETCH_SIZE 265 300 KiB x 2.00 + WRITE_SIZE 32 863 KiB x 1.00 per
+# dispatch, the factors being what the counters read for a 256 MiB device
+# copy (profiles/glm_grad_mfma_logistic_65536x256_counters.md)
+C2_HBM_BYTES_PER_GRADIENT = (265300 * 2.0 + 32863 * 1.0) * 1024
 def logistic_65536x256(args, torch, pa):
   from probability_amd import workloads
   n, d, c, L = 65536, 256, 4096, 20
@@ -276,8 +281,12 @@ def logistic_65536x256(args, torch, pa):
       'chain_grad_evals_per_s': c * (nb + nr) * L / (ms / 1e3),
       'roofline': {'bound': 'mfma', 'achieved': achieved,
                    'peak': BF16_PEAK_TFLOPS, 'unit': 'TFLOP/s',
-                   'frac': achieved / BF16_PEAK_TFLOPS, 'traffic': None,
-                   'kernel': 'glm_grad_mfma + mfma_chain_step (sampling '
+                   'frac': achieved / BF16_PEAK_TFLOPS,
+                   'traffic': C2_HBM_BYTES_PER_GRADIENT,
+                   'traffic_what': 'HBM bytes per gradient launch '
+                                   '(FETCH_SIZE + WRITE_SIZE, '
+                                   'copy-calibrated); X and X^T are 64 MiB',
+                   'kernel': 'glm_grad_mfma<8> + mfma_chain_step (sampling '
                              'launch, all 1000 transitions)',
                    'algorithmic_flop_per_gradient': flop_grad,
                    'executed_mfma_flop_per_gradient': 2 * flop_grad,
