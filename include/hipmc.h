@@ -60,7 +60,7 @@
 extern "C" {
 #endif
 
-#define HIPMC_ABI_VERSION 3
+#define HIPMC_ABI_VERSION 4
 
 /* error codes */
 #define HIPMC_OK 0
@@ -377,7 +377,11 @@ int hipmc_target_eval(int32_t dtype, int32_t target_kind,
  * chains' log accept probabilities with reduce_logmeanexp (the class
  * default), HIPMC_REDUCE_HARMONIC with reduce_log_harmonic_mean_exp
  * (math/generic.py:387-432), as windowed adaptation configures it
- * (experimental/mcmc/windowed_sampling.py:869-873).
+ * (experimental/mcmc/windowed_sampling.py:869-873).  elem_step and
+ * elem_num_adaptation_steps (int64 [n_steps], or NULL for the scalars
+ * `step` and `num_adaptation_steps`) give each element its own DA step
+ * count and adaptation length, so problems at different points of their
+ * adaptation (requests admitted at different times) update in one launch.
  */
 int hipmc_da_update(int32_t dtype, int32_t n_steps, const int32_t* step_group,
                     const int32_t* state_index,
@@ -388,7 +392,8 @@ int hipmc_da_update(int32_t dtype, int32_t n_steps, const int32_t* step_group,
                     const double* target_accept_prob, int64_t step,
                     int64_t num_adaptation_steps, double exploration_shrinkage,
                     double step_count_smoothing, double decay_rate,
-                    int32_t reduce_mode, void* step_size,
+                    int32_t reduce_mode, const int64_t* elem_step,
+                    const int64_t* elem_num_adaptation_steps, void* step_size,
                     double* new_step_size_f64, uint64_t stream);
 
 /*

@@ -131,7 +131,8 @@ def lib():
                                       _vp, _vp, _vp, _u64]
       l.hipmc_da_update.argtypes = [_i32, _i32, _vp, _vp, _vp, _vp, _i32,
                                     _i64, _vp, _vp, _vp, _vp, _i64, _i64,
-                                    _f64, _f64, _f64, _i32, _vp, _vp, _u64]
+                                    _f64, _f64, _f64, _i32, _vp, _vp, _vp,
+                                    _vp, _u64]
       l.hipmc_variance_update.argtypes = [_i32, _i32, _vp, _vp, _vp, _i32,
                                           _i32, _vp, _vp, _vp, _u64]
       l.hipmc_simple_step_update.argtypes = [_i32, _i32, _vp, _vp, _vp, _i32,

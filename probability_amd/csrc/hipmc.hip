@@ -182,11 +182,15 @@ __global__ void __launch_bounds__(256)
               const int32_t* group_start,
               const T* lar_base, int n_rows, int64_t row_stride,
               double* error_sum, double* log_avg, const double* log_shrink,
-              const double* target_accept, int64_t step0, int64_t num_adapt,
+              const double* target_accept, int64_t step_all,
+              int64_t num_adapt_all, const int64_t* elem_step,
+              const int64_t* elem_num_adapt,
               double gamma, double t0, double kappa, T* step_size,
               double* step_f64, int harmonic) {
   __shared__ double red[256];
   const int elem = blockIdx.x;
+  const int64_t step0 = elem_step ? elem_step[elem] : step_all;
+  const int64_t num_adapt = elem_num_adapt ? elem_num_adapt[elem] : num_adapt_all;
   const int grp = step_group ? step_group[elem] : elem;
   const int gidx = state_index ? state_index[elem] : elem;
   const int b = group_start[grp], e = group_start[grp + 1];
@@ -657,7 +661,8 @@ int hipmc_da_update(int32_t dtype, int32_t n_groups, const int32_t* step_group,
                     const double* target_accept_prob, int64_t step,
                     int64_t num_adaptation_steps, double exploration_shrinkage,
                     double step_count_smoothing, double decay_rate,
-                    int32_t reduce_mode, void* step_size,
+                    int32_t reduce_mode, const int64_t* elem_step,
+                    const int64_t* elem_num_adapt, void* step_size,
                     double* new_step_size_f64, uint64_t stream) {
   if (reduce_mode != HIPMC_REDUCE_MEAN && reduce_mode != HIPMC_REDUCE_HARMONIC)
     return fail(HIPMC_E_INVALID, "da_update: bad reduce_mode");
@@ -674,6 +679,7 @@ int hipmc_da_update(int32_t dtype, int32_t n_groups, const int32_t* step_group,
                        row_stride, error_sum,
                        log_averaging_step, log_shrinkage_target,
                        target_accept_prob, step, num_adaptation_steps,
+                       elem_step, elem_num_adapt,
                        exploration_shrinkage, step_count_smoothing, decay_rate,
                        (float*)step_size, new_step_size_f64, reduce_mode);
   else if (dtype == HIPMC_F64)
@@ -683,6 +689,7 @@ int hipmc_da_update(int32_t dtype, int32_t n_groups, const int32_t* step_group,
                        row_stride, error_sum,
                        log_averaging_step, log_shrinkage_target,
                        target_accept_prob, step, num_adaptation_steps,
+                       elem_step, elem_num_adapt,
                        exploration_shrinkage, step_count_smoothing, decay_rate,
                        (double*)step_size, new_step_size_f64, reduce_mode);
   else

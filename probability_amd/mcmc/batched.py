@@ -541,7 +541,8 @@ class _Run:
         _lib.ptr(b.group_start), _lib.ptr(self.lar[t]), 1, b.n_chains,
         _lib.ptr(self.err), _lib.ptr(self.avg), _lib.ptr(self.lst),
         _lib.ptr(self.tgt), w.da_step, w.da_num, 0.05, 10.0, 0.75,
-        _lib.REDUCE_HARMONIC, _lib.ptr(self.step), _lib.ptr(self.step64),
+        _lib.REDUCE_HARMONIC, None, None, _lib.ptr(self.step),
+        _lib.ptr(self.step64),
         _lib.stream_handle(b.device)))
     w.da_step += 1
     w.dm_step += 1
@@ -621,7 +622,8 @@ class _Run:
         _lib.ptr(b.group_start), _lib.ptr(self.lar[t0]), n, b.n_chains,
         _lib.ptr(self.err), _lib.ptr(self.avg), _lib.ptr(self.lst),
         _lib.ptr(self.tgt), t0, self.num_adapt, 0.05, 10.0, 0.75,
-        _lib.REDUCE_MEAN, _lib.ptr(self.step), _lib.ptr(self.step64),
+        _lib.REDUCE_MEAN, None, None, _lib.ptr(self.step),
+        _lib.ptr(self.step64),
         _lib.stream_handle(b.device)))
 
   def finish(self, consumer=None):

@@ -227,7 +227,8 @@ class DualAveragingStepSizeAdaptation(kernel_base.TransitionKernel):
         _lib.ptr(avg), _lib.ptr(lst), _lib.ptr(tgt), step_no,
         int(pkr.num_adaptation_steps), float(pkr.exploration_shrinkage),
         float(pkr.step_count_smoothing), float(pkr.decay_rate),
-        reduce_mode(self.parameters['reduce_fn']), _lib.ptr(new_step),
+        reduce_mode(self.parameters['reduce_fn']), None, None,
+        _lib.ptr(new_step),
         _lib.ptr(step64), _lib.stream_handle(device)))
     return new_step.reshape(shape), err.reshape(shape), avg.reshape(shape)
 

@@ -336,7 +336,8 @@ def _run_fused(plan, num_results, current_state, num_burnin_steps, thin,
           _lib.ptr(err), _lib.ptr(avg), _lib.ptr(lst), _lib.ptr(tgt),
           da0 + t, n_adapt, float(pkr.exploration_shrinkage),
           float(pkr.step_count_smoothing), float(pkr.decay_rate),
-          da_lib.reduce_mode(plan.da.parameters['reduce_fn']), _lib.ptr(step),
+          da_lib.reduce_mode(plan.da.parameters['reduce_fn']), None, None,
+          _lib.ptr(step),
           _lib.ptr(step64), stream))
     t += n
   if stats is not None:

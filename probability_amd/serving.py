@@ -153,6 +153,67 @@ class _Lane:
     self.adapting = adapting
     self.stream = torch.cuda.Stream(device=device)
     self.inflight = None
+    # device tables of the lane's current set of requests (rebuilt when a
+    # request joins or leaves the lane)
+    self.tables = None
+
+
+# hipmc_stream as a NumPy record (one row per request of a launch)
+_STREAM_DTYPE = np.dtype([
+    ('seed', '<u4', (2,)), ('iter0', '<i8'), ('n_iters', '<i4'),
+    ('trace_every', '<i4'), ('trace', '<u8'), ('trace_start', '<i8'),
+    ('n_trace', '<i4'), ('pad_', '<i4')])
+assert _STREAM_DTYPE.itemsize == ctypes.sizeof(_lib.Stream)
+
+
+class _LaneTables:
+  """Device data of one set of requests in a lane: the problem table, the
+  per-chain maps (problem, local index, pool row, step-size slot), the DA
+  group table and slots, and the host template of the per-request Philox
+  streams.  Uploaded once per set, in one copy."""
+
+  def __init__(self, reqs, device, stream):
+    p = len(reqs)
+    raw = bytearray()
+    cp, cl, rows, cs = [], [], [], []
+    offs = [0]
+    st = np.zeros(p, _STREAM_DTYPE)
+    for i, r in enumerate(reqs):
+      prob = _lib.Problem.from_buffer_copy(r.problem_bytes)
+      prob.n_chains = r.num_chains
+      prob.chain_offset = offs[-1]
+      prob.key = r.key
+      raw += bytes(prob)
+      cp += [i] * r.num_chains
+      cl += list(range(r.num_chains))
+      rows += r.rows
+      cs += [r.da_slot] * r.num_chains
+      offs.append(offs[-1] + r.num_chains)
+      st['seed'][i] = (int(r.seed[0]), int(r.seed[1]))
+      st['trace_every'][i] = 1
+      st['trace'][i] = r.draws.data_ptr()
+      st['trace_start'][i] = r.num_adaptation_steps
+      st['n_trace'][i] = r.num_results
+    n = offs[-1]
+    ints = np.concatenate([np.asarray(v, np.int32) for v in
+                           (cp, cl, rows, cs, offs,
+                            [r.da_slot for r in reqs])])
+    blob = bytes(raw) + ints.tobytes()
+    with torch.cuda.stream(stream):
+      self.buf = torch.frombuffer(bytearray(blob), dtype=torch.uint8).to(
+          device, non_blocking=True)
+    base = self.buf.data_ptr()
+    self.p, self.n = p, n
+    self.table = base
+    o = base + len(raw)
+    self.chain_problem, self.chain_local = o, o + 4 * n
+    self.chain_row, self.chain_step = o + 8 * n, o + 12 * n
+    self.group_start = o + 16 * n
+    self.slots = self.group_start + 4 * (p + 1)
+    self.streams = st
+    self.adapt_len = np.asarray([r.num_adaptation_steps for r in reqs],
+                                np.int64)
+    self.reqs = reqs
 
 
 class SamplerServer:
@@ -260,43 +321,30 @@ class SamplerServer:
     return admitted
 
   # ------------------------------------------------------------ launches
-  def _launch(self, pool, reqs, adapting, stream):
+  def _launch(self, pool, reqs, adapting, lane):
     if not reqs:
       return None
     dt = self.dtype
     dev = self.device
-    p = len(reqs)
-    raw = bytearray()
-    cp, cl, rows, cs = [], [], [], []
-    sarr = (_lib.Stream * p)()
-    offs = [0]
-    for i, r in enumerate(reqs):
-      prob = _lib.Problem.from_buffer_copy(r.problem_bytes)
-      prob.n_chains = r.num_chains
-      prob.chain_offset = offs[-1]
-      prob.key = r.key
-      raw += bytes(prob)
-      cp += [i] * r.num_chains
-      cl += list(range(r.num_chains))
-      rows += r.rows
-      cs += [r.da_slot] * r.num_chains
-      offs.append(offs[-1] + r.num_chains)
-      st = sarr[i]
-      st.seed[0], st.seed[1] = int(r.seed[0]), int(r.seed[1])
-      st.iter0 = r.iter
-      st.n_iters = 1 if adapting else min(self.iters_per_tick,
-                                          r.total_steps - r.iter)
-      st.trace_every = 1
-      st.trace = r.draws.data_ptr()
-      st.trace_start = r.num_adaptation_steps
-      st.n_trace = r.num_results
-    n = offs[-1]
+    stream = lane.stream
+    tb = lane.tables
+    if tb is None or tb.reqs != reqs:
+      tb = lane.tables = _LaneTables(reqs, dev, stream)
+    p, n = tb.p, tb.n
     n_iters = 1 if adapting else self.iters_per_tick
+    # per launch: each request's iteration offset and count, and (adapting)
+    # its DA step count and adaptation length: one upload
+    st = tb.streams
+    st['iter0'] = [r.iter for r in reqs]
+    st['n_iters'] = [1 if adapting else min(self.iters_per_tick,
+                                            r.total_steps - r.iter)
+                     for r in reqs]
+    blob = st.tobytes()
+    if adapting:
+      blob += np.asarray([r.iter for r in reqs], np.int64).tobytes()
+      blob += tb.adapt_len.tobytes()
     with torch.cuda.stream(stream):
-      table = torch.frombuffer(raw, dtype=torch.uint8).to(dev, non_blocking=True)
-      streams_t = torch.frombuffer(bytearray(bytes(sarr)), dtype=torch.uint8
-                                   ).to(dev, non_blocking=True)
-      ints = torch.as_tensor(np.array([cp, cl, rows, cs], np.int32)).to(
+      up = torch.frombuffer(bytearray(blob), dtype=torch.uint8).to(
           dev, non_blocking=True)
       lar = torch.empty((n_iters, n), dtype=dt, device=dev)
       a = _lib.RunArgs()
@@ -312,53 +360,27 @@ class SamplerServer:
                        else _lib.TARGET_LOGISTIC)
       a.waves_per_chain = reqs[0].waves
       a.max_energy_diff = 1000.0
-      a.problems = table.data_ptr()
+      a.problems = tb.table
       a.n_problems = p
-      a.chain_problem = ints[0].data_ptr()
-      a.chain_local = ints[1].data_ptr()
-      a.chain_row = ints[2].data_ptr()
-      a.chain_step = ints[3].data_ptr()
+      a.chain_problem, a.chain_local = tb.chain_problem, tb.chain_local
+      a.chain_row, a.chain_step = tb.chain_row, tb.chain_step
       a.step_size = pool.step.data_ptr()
       a.state, a.tlp, a.grad = (pool.state.data_ptr(), pool.tlp.data_ptr(),
                                 pool.grad.data_ptr())
       a.out_log_accept_ratio = lar.data_ptr()
-      a.streams = streams_t.data_ptr()
+      a.streams = up.data_ptr()
       _lib.check(_lib.lib().hipmc_nuts_run(a, stream.cuda_stream))
       if adapting:
-        gstart = torch.as_tensor(np.asarray(offs, np.int32)).to(
-            dev, non_blocking=True)
-        slots = torch.as_tensor(np.asarray([r.da_slot for r in reqs],
-                                           np.int32)).to(dev, non_blocking=True)
-        key0 = (reqs[0].iter, reqs[0].num_adaptation_steps)
-        if any((r.iter, r.num_adaptation_steps) != key0 for r in reqs):
-          # requests at different DA steps: one update per distinct step
-          groups = collections.defaultdict(list)
-          for i, r in enumerate(reqs):
-            groups[(r.iter, r.num_adaptation_steps)].append(i)
-          for (step_no, _), idx in groups.items():
-            sub = torch.as_tensor(np.asarray(idx, np.int32)).to(
-                dev, non_blocking=True)
-            self._da(pool, reqs[idx[0]], sub, slots, gstart, lar, n,
-                     step_no, stream)
-        else:
-          self._da(pool, reqs[0], None, slots, gstart, lar, n, key0[0],
-                   stream)
-      keep = (table, streams_t, ints, lar)
-    return keep
-
-  def _da(self, pool, req, elems, slots, gstart, lar, n, step_no, stream):
-    if elems is None:
-      n_el = slots.numel()
-      sg, si = None, slots
-    else:
-      n_el = elems.numel()
-      sg, si = elems, slots.index_select(0, elems.long()).to(torch.int32)
-    _lib.check(_lib.lib().hipmc_da_update(
-        _lib.dtype_code(self.dtype), n_el, _lib.ptr(sg), _lib.ptr(si),
-        _lib.ptr(gstart), _lib.ptr(lar), 1, n, _lib.ptr(pool.err),
-        _lib.ptr(pool.avg), _lib.ptr(pool.lst), _lib.ptr(pool.tgt), step_no,
-        req.num_adaptation_steps, 0.05, 10.0, 0.75, _lib.REDUCE_MEAN,
-        _lib.ptr(pool.step), _lib.ptr(pool.step64), stream.cuda_stream))
+        # every request's DA step in one launch, each at its own step count
+        # and adaptation length
+        es = up.data_ptr() + st.nbytes
+        _lib.check(_lib.lib().hipmc_da_update(
+            _lib.dtype_code(dt), p, None, tb.slots, tb.group_start,
+            _lib.ptr(lar), 1, n, _lib.ptr(pool.err), _lib.ptr(pool.avg),
+            _lib.ptr(pool.lst), _lib.ptr(pool.tgt), 0, 0, 0.05, 10.0, 0.75,
+            _lib.REDUCE_MEAN, es, es + 8 * p, _lib.ptr(pool.step),
+            _lib.ptr(pool.step64), stream.cuda_stream))
+    return (tb, up, lar)
 
   def tick(self):
     """Admits what fits, then for every lane whose launch has finished
@@ -393,7 +415,7 @@ class SamplerServer:
     if not reqs:
       return False
     pool = self.pools[lane.pool_key]
-    keep = self._launch(pool, reqs, lane.adapting, lane.stream)
+    keep = self._launch(pool, reqs, lane.adapting, lane)
     ev = torch.cuda.Event()
     ev.record(lane.stream)
     for r in reqs:

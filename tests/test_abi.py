@@ -32,7 +32,7 @@ def test_struct_layouts():
   assert lib.hipmc_sizeof_problem() == ctypes.sizeof(_lib.Problem)
   assert lib.hipmc_sizeof_run_args() == ctypes.sizeof(_lib.RunArgs)
   assert lib.hipmc_sizeof_stream() == ctypes.sizeof(_lib.Stream)
-  assert lib.hipmc_abi_version() == 3
+  assert lib.hipmc_abi_version() == 4
 
 
 def test_invalid_arguments_return_codes_without_device():

@@ -76,10 +76,56 @@ def test_da_harmonic_matches_oracle():
       _lib.dtype_code(torch.float64), 3, None, None,
       _lib.ptr(gstart), _lib.ptr(t_lar), rows, c, _lib.ptr(err), _lib.ptr(avg),
       _lib.ptr(lst), _lib.ptr(tgt), 0, 10, 0.05, 10.0, 0.75,
-      _lib.REDUCE_HARMONIC, _lib.ptr(step), _lib.ptr(step64), 0))
+      _lib.REDUCE_HARMONIC, None, None, _lib.ptr(step), _lib.ptr(step64),
+      0))
   np.testing.assert_allclose(_np(step), st.step_size, rtol=1e-12)
   np.testing.assert_allclose(_np(err), st.error_sum, rtol=1e-12, atol=1e-14)
   np.testing.assert_allclose(_np(avg), st.log_averaging_step, rtol=1e-12)
+
+
+def test_da_update_with_per_element_step_counts():
+  # elements at different DA steps and adaptation lengths in one launch
+  # equal one launch per element with the scalar step and length
+  r = np.random.default_rng(4)
+  sizes = [4, 6, 5]
+  c = sum(sizes)
+  lar = np.minimum(0, r.normal(-0.4, 0.5, size=c))
+  steps, lens = [0, 7, 3], [10, 8, 3]
+  dev = 'cuda'
+  gstart = torch.as_tensor(np.concatenate([[0], np.cumsum(sizes)]).astype(
+      np.int32), device=dev)
+  t_lar = torch.as_tensor(lar, device=dev)
+
+  def fresh():
+    step = torch.as_tensor([0.3, 0.05, 1.2], dtype=torch.float64,
+                           device=dev)
+    err = torch.as_tensor([0.0, 0.2, -0.1], dtype=torch.float64, device=dev)
+    avg = torch.as_tensor([0.0, -1.0, 0.3], dtype=torch.float64, device=dev)
+    lst = np.log(10.0) + torch.log(step)
+    tgt = torch.full((3,), 0.8, dtype=torch.float64, device=dev)
+    return step, step.clone(), err, avg, lst, tgt
+
+  step, step64, err, avg, lst, tgt = fresh()
+  es = torch.as_tensor(steps, dtype=torch.int64, device=dev)
+  el = torch.as_tensor(lens, dtype=torch.int64, device=dev)
+  _lib.check(_lib.lib().hipmc_da_update(
+      _lib.dtype_code(torch.float64), 3, None, None, _lib.ptr(gstart),
+      _lib.ptr(t_lar), 1, c, _lib.ptr(err), _lib.ptr(avg), _lib.ptr(lst),
+      _lib.ptr(tgt), 0, 0, 0.05, 10.0, 0.75, _lib.REDUCE_MEAN, _lib.ptr(es),
+      _lib.ptr(el), _lib.ptr(step), _lib.ptr(step64), 0))
+  for k in range(3):
+    s2, s64, e2, a2, l2, t2 = fresh()
+    idx = torch.as_tensor([k], dtype=torch.int32, device=dev)
+    _lib.check(_lib.lib().hipmc_da_update(
+        _lib.dtype_code(torch.float64), 1, _lib.ptr(idx), _lib.ptr(idx),
+        _lib.ptr(gstart), _lib.ptr(t_lar), 1, c, _lib.ptr(e2), _lib.ptr(a2),
+        _lib.ptr(l2), _lib.ptr(t2), steps[k], lens[k], 0.05, 10.0, 0.75,
+        _lib.REDUCE_MEAN, None, None, _lib.ptr(s2), _lib.ptr(s64), 0))
+    assert float(step[k]) == float(s2[k])
+    assert float(err[k]) == float(e2[k])
+    assert float(avg[k]) == float(a2[k])
+  # the element past its adaptation (step 3 of 3) keeps its step size
+  assert float(step[2]) == 1.2
 
 
 @pytest.mark.parametrize('tdt', [torch.float32, torch.float64])
