@@ -500,8 +500,12 @@ def _ragged_roofline(specs, stats, run_s):
          'kernel': 'nuts_kernel<float,3,glm> (all size classes)',
          'what': 'design-matrix bytes read per second: N_k D_k 4 B per '
                  'chain-leapfrog, summed over the run, over its wall time '
-                 '(about 95% of them hit L2; the kernel is bound by L2 '
-                 'bandwidth and per-wave load latency)',
+                 '(about 95% of them hit L2)',
+         'limiter': 'FP32 VALU issue in the gradient passes of the '
+                    'largest size class (timed: 0.19 us per '
+                    'chain-leapfrog at 8192 x 128 with 2048 chains, '
+                    '22 TFLOP/s, whatever the waves per chain or chain '
+                    'groups)',
          'chain_leapfrogs': float(lpp.sum()),
          'design_matrix_entries': entries,
          'traffic': None}
