@@ -162,19 +162,22 @@ __device__ inline A row_sum(A v) {
   v = dpp_add<kDppHalfMirror>(v);
   return dpp_add<kDppMirror>(v);
 }
-// v of lane i of this lane's quad (DPP quad_perm [i, i, i, i]).
+// v of lane i of this lane's quad (DPP quad_perm [i, i, i, i]).  Every lane
+// reads a lane of its own quad, so no lane keeps its old value: the move
+// carries no old operand, which lets the compiler fold it into the
+// instructions that use it.
 template <int I>
 __device__ inline float quad_bcast(float v) {
-  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v),
-                                                    I * 0x55, 0xF, 0xF, false));
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), I * 0x55,
+                                                 0xF, 0xF, false));
 }
 template <int I>
 __device__ inline double quad_bcast(double v) {
   const int64_t b = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, I * 0x55,
-                                             0xF, 0xF, false);
-  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32),
-                                             I * 0x55, 0xF, 0xF, false);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(uint32_t)b, I * 0x55, 0xF,
+                                          0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(uint32_t)(b >> 32), I * 0x55,
+                                          0xF, 0xF, false);
   return __longlong_as_double(((int64_t)(uint32_t)hi << 32) | (uint32_t)lo);
 }
 // The same for an index that is a constant after unrolling.

@@ -660,6 +660,18 @@ struct Target {
     auto row0 = [&](int q) { return q0 + (q / NQ) * qs + (q % NQ) * RB; };
     auto own = [&](int k) { return B >= 4 ? (lane & 3) + 4 * k : (lane & 1); };
     auto load = [&](int r0, VT (&xv)[B][NBB], T (&yv)[RPL]) {
+      if (r0 + RB <= n) {
+        // the whole batch lies inside X: rows 4 apart from one base
+        const gptr<T> xr = x + (size_t)(r0 + ph) * xp;
+#pragma unroll
+        for (int u = 0; u < B; ++u)
+#pragma unroll
+          for (int b = 0; b < NBB; ++b)
+            xv[u][b] = *(gptr<VT>)(xr + (size_t)(4 * u) * xp + cst[b]);
+#pragma unroll
+        for (int k = 0; k < RPL; ++k) yv[k] = y[r0 + 4 * own(k) + ph];
+        return;
+      }
 #pragma unroll
       for (int u = 0; u < B; ++u) {
         const int j = r0 + 4 * u + ph;
@@ -674,17 +686,8 @@ struct Target {
         yv[k] = y[j < n ? j : n - 1];
       }
     };
-    int r0 = row0(0);
-    if (r0 >= n) return;
-    VT xv[B][NBB];
-    T yv[RPL];
-    load(r0, xv, yv);
-    for (int q = 1;; ++q) {
-      const int r1 = row0(q);
-      const bool more = r1 < n;
-      VT nx[B][NBB];
-      T ny[RPL];
-      if (more) load(r1, nx, ny);
+    // one batch: logits, residuals and log-likelihood terms, gradient
+    auto process = [&](int r0, const VT (&xv)[B][NBB], const T (&yv)[RPL]) {
       T sr[B];
 #pragma unroll
       for (int u = 0; u < B; ++u) {
@@ -728,14 +731,26 @@ struct Target {
       for (int u = 0; u < B; ++u)
 #pragma unroll
         for (int b = 0; b < NBB; ++b) acc[b] += xv[u][b] * rj[u];
-      if (!more) break;
-      r0 = r1;
-#pragma unroll
-      for (int u = 0; u < B; ++u)
-#pragma unroll
-        for (int b = 0; b < NBB; ++b) xv[u][b] = nx[u][b];
-#pragma unroll
-      for (int k = 0; k < RPL; ++k) yv[k] = ny[k];
+    };
+    int r0 = row0(0);
+    if (r0 >= n) return;
+    // two register buffers used in turn (the next batch loads into one while
+    // the other is processed), unrolled by two so that no copies are needed
+    VT xa[B][NBB], xb[B][NBB];
+    T ya[RPL], yb[RPL];
+    load(r0, xa, ya);
+    for (int q = 1;; q += 2) {
+      const int r1 = row0(q);
+      const bool m1 = r1 < n;
+      if (m1) load(r1, xb, yb);
+      process(r0, xa, ya);
+      if (!m1) break;
+      const int r2 = row0(q + 1);
+      const bool m2 = r2 < n;
+      if (m2) load(r2, xa, ya);
+      process(r1, xb, yb);
+      if (!m2) break;
+      r0 = r2;
     }
   }
 
@@ -930,6 +945,17 @@ struct Target {
     const int q0 = grp_w * CH, qs = grp_waves * CH;
     auto row0 = [&](int q) { return q0 + (q / NQ) * qs + (q % NQ) * RB; };
     auto load = [&](int r0, VT (&xv)[B][NBB], T& yv) {
+      if (r0 + RB <= n) {
+        // the whole batch lies inside X: rows 4 apart from one base
+        const gptr<T> xr = x + (size_t)(r0 + ph) * xp;
+#pragma unroll
+        for (int u = 0; u < B; ++u)
+#pragma unroll
+          for (int b = 0; b < NBB; ++b)
+            xv[u][b] = *(gptr<VT>)(xr + (size_t)(4 * u) * xp + cst[b]);
+        yv = y[r0 + 4 * (lane & 1) + ph];
+        return;
+      }
 #pragma unroll
       for (int u = 0; u < B; ++u) {
         const int j = r0 + 4 * u + ph;
@@ -941,17 +967,8 @@ struct Target {
       const int jr = r0 + 4 * (lane & 1) + ph;
       yv = y[jr < n ? jr : n - 1];
     };
-    int r0 = row0(0);
-    if (r0 >= n) return;
-    VT xv[B][NBB];
-    T yv;
-    load(r0, xv, yv);
-    for (int q = 1;; ++q) {
-      const int r1 = row0(q);
-      const bool more = r1 < n;
-      VT nx[B][NBB];
-      T ny;
-      if (more) load(r1, nx, ny);
+    // one batch for every chain of the group
+    auto process = [&](int r0, const VT (&xv)[B][NBB], T yv) {
       const bool valid = r0 + 4 * (lane & 1) + ph < n;
       const bool add = valid && counts;
 #pragma unroll
@@ -989,13 +1006,25 @@ struct Target {
           for (int b = 0; b < NBB; ++b) acc[j][b] += xv[u][b] * rj;
         }
       }
-      if (!more) break;
-      r0 = r1;
-#pragma unroll
-      for (int u = 0; u < B; ++u)
-#pragma unroll
-        for (int b = 0; b < NBB; ++b) xv[u][b] = nx[u][b];
-      yv = ny;
+    };
+    int r0 = row0(0);
+    if (r0 >= n) return;
+    // two register buffers used in turn, unrolled by two (no copies)
+    VT xa[B][NBB], xb[B][NBB];
+    T ya, yb;
+    load(r0, xa, ya);
+    for (int q = 1;; q += 2) {
+      const int r1 = row0(q);
+      const bool m1 = r1 < n;
+      if (m1) load(r1, xb, yb);
+      process(r0, xa, ya);
+      if (!m1) break;
+      const int r2 = row0(q + 1);
+      const bool m2 = r2 < n;
+      if (m2) load(r2, xa, ya);
+      process(r1, xb, yb);
+      if (!m2) break;
+      r0 = r2;
     }
   }
 
