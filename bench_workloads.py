@@ -486,26 +486,30 @@ def ragged_glms_512(args, torch, pa):
 
 
 def _ragged_roofline(specs, stats, run_s):
-  """Design-matrix bytes streamed per second over one run: every
-  chain-leapfrog reads its problem's X once (N_k D_k 4 bytes; the
-  per-chain evaluation), summed over the run's leapfrogs, over the run's
+  """The GLM kernel's arithmetic over one run: every chain-leapfrog takes
+  its problem's logits and gradient in one pass over X, two FMAs (4 FLOP)
+  per design-matrix entry, summed over the run's leapfrogs, over the run's
   wall time (the size classes' kernels overlap on their streams and cover
-  nearly all of it)."""
+  nearly all of it), against the FP32 peak.  The kernel is compute-bound:
+  timed alone, chain groups that read X once for four chains run at the
+  same rate per chain-leapfrog as one chain per block.  The design-matrix
+  bytes those leapfrogs read (N_k D_k 4 B each, ~95% from L2) are reported
+  beside it, with the HBM traffic from counters."""
   lpp = stats['leapfrogs_per_problem']
   entries = float(sum(float(lpp[i]) * s.n_obs * s.dim
                       for i, s in enumerate(specs)))
-  achieved = entries * 4.0 / run_s / 1e9
-  out = {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS,
-         'unit': 'GB/s', 'frac': achieved / HBM_PEAK_GBS,
+  achieved = 4.0 * entries / run_s / 1e12
+  out = {'bound': 'mfma', 'achieved': achieved, 'peak': FP32_PEAK_TFLOPS,
+         'unit': 'TFLOP/s', 'frac': achieved / FP32_PEAK_TFLOPS,
          'kernel': 'nuts_kernel<float,3,glm> (all size classes)',
-         'what': 'design-matrix bytes read per second: N_k D_k 4 B per '
-                 'chain-leapfrog, summed over the run, over its wall time '
-                 '(about 95% of them hit L2)',
-         'limiter': 'FP32 VALU issue in the gradient passes of the '
-                    'largest size class (timed: 0.19 us per '
-                    'chain-leapfrog at 8192 x 128 with 2048 chains, '
-                    '22 TFLOP/s, whatever the waves per chain or chain '
-                    'groups)',
+         'what': 'GLM logit and gradient FLOP (4 per design-matrix entry '
+                 'per chain-leapfrog) over the run\'s wall time',
+         'limiter': 'FP32 VALU work of the gradient passes (FMAs, DPP row '
+                    'sums, the sigmoid) with no MFMA and packed FP32 off; '
+                    'timed alone at 8192 x 128 with 2048 chains: 0.19 us '
+                    'per chain-leapfrog, 22 TFLOP/s, with or without chain '
+                    'groups',
+         'design_matrix_GBps': entries * 4.0 / run_s / 1e9,
          'chain_leapfrogs': float(lpp.sum()),
          'design_matrix_entries': entries,
          'traffic': None}
