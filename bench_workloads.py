@@ -506,8 +506,8 @@ def _ragged_roofline(specs, stats, run_s):
                  'per chain-leapfrog) over the run\'s wall time',
          'limiter': 'FP32 VALU work of the gradient passes (FMAs, DPP row '
                     'sums, the sigmoid) with no MFMA and packed FP32 off; '
-                    'timed alone at 8192 x 128 with 2048 chains: 0.19 us '
-                    'per chain-leapfrog, 22 TFLOP/s, with or without chain '
+                    'timed alone at 8192 x 128 with 2048 chains: 0.17 us '
+                    'per chain-leapfrog, 24 TFLOP/s, with or without chain '
                     'groups',
          'design_matrix_GBps': entries * 4.0 / run_s / 1e9,
          'chain_leapfrogs': float(lpp.sum()),

@@ -203,6 +203,68 @@ __device__ inline A rows_total(A v) {
 
 // Sum over all 64 lanes; every lane receives the (wave-uniform) result.
 // Requires a full EXEC mask.  Deterministic: fixed reduction order.
+// src read through a DPP pattern into the lanes of RowMask x BankMask (a
+// bank is 4 lanes of a row); the other lanes keep `old`.
+template <int Ctrl, int BankMask>
+__device__ inline float dpp_mov(float old, float src) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(
+      __float_as_int(old), __float_as_int(src), Ctrl, 0xF, BankMask, false));
+}
+template <int Ctrl, int BankMask>
+__device__ inline double dpp_mov(double old, double src) {
+  const int64_t o = __double_as_longlong(old), b = __double_as_longlong(src);
+  const int lo = __builtin_amdgcn_update_dpp((int)(uint32_t)o, (int)(uint32_t)b,
+                                             Ctrl, 0xF, BankMask, false);
+  const int hi = __builtin_amdgcn_update_dpp((int)(uint32_t)(o >> 32),
+                                             (int)(uint32_t)(b >> 32), Ctrl,
+                                             0xF, BankMask, false);
+  return __longlong_as_double(((int64_t)(uint32_t)hi << 32) | (uint32_t)lo);
+}
+// x + src[perm(lane)] inside each row of 16 lanes.
+template <int Ctrl>
+__device__ inline float dpp_add_from(float x, float src) {
+  return x + __int_as_float(__builtin_amdgcn_update_dpp(
+                 0, __float_as_int(src), Ctrl, 0xF, 0xF, true));
+}
+template <int Ctrl>
+__device__ inline double dpp_add_from(double x, double src) {
+  const int64_t b = __double_as_longlong(src);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)b, Ctrl, 0xF,
+                                             0xF, true);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(uint32_t)(b >> 32), Ctrl,
+                                             0xF, 0xF, true);
+  return x + __longlong_as_double(((int64_t)(uint32_t)hi << 32) |
+                                  (uint32_t)lo);
+}
+constexpr int kDppRowShl4 = 0x104;   // lane i reads lane i + 4 (in its row)
+constexpr int kDppRowShr4 = 0x114;   // lane i reads lane i - 4
+constexpr int kDppRowRor8 = 0x128;   // lane i reads lane i ^ 8 (halves swap)
+
+// The row sums of four values a0..a3 (the partial dot products of four rows
+// of a batch), delivered so that lane L holds the sum over its 16-lane DPP
+// row of a_{L & 3}: bit for bit row_sum(a_{L & 3}) (the same pairs are
+// added in the same order: lane pairs L^1, L^2, then the quads 0+1 / 2+3,
+// then the halves; float adds are commutative), in 13 instructions instead
+// of four row_sum (16) and a pick.  b0, b1: bits 0 and 1 of the lane.
+template <typename A>
+__device__ inline A row_sum_pick4(A a0, A a1, A a2, A a3, bool b0, bool b1) {
+  // pairs L, L^1: keep rows L&1 and 2 + (L&1), send the partner's
+  A x = b0 ? a1 : a0, sx = b0 ? a0 : a1;
+  A y = b0 ? a3 : a2, sy = b0 ? a2 : a3;
+  x = dpp_add_from<kDppQuadXor1>(x, sx);
+  y = dpp_add_from<kDppQuadXor1>(y, sy);
+  // pairs L, L^2: keep row L & 3
+  A k = b1 ? y : x;
+  const A s = b1 ? x : y;
+  k = dpp_add_from<kDppQuadXor2>(k, s);
+  // quads 0+1 and 2+3 (lane L with L ^ 4, the same row)
+  A t = dpp_mov<kDppRowShl4, 0x5>(k, k);
+  t = dpp_mov<kDppRowShr4, 0xA>(t, k);
+  k = k + t;
+  // halves (lane L with L ^ 8)
+  return dpp_add_from<kDppRowRor8>(k, k);
+}
+
 __device__ inline float wave_sum(float v) { return rows_total(row_sum(v)); }
 __device__ inline double wave_sum(double v) { return rows_total(row_sum(v)); }
 

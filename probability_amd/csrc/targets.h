@@ -688,7 +688,7 @@ struct Target {
     };
     // one batch: logits, residuals and log-likelihood terms, gradient
     auto process = [&](int r0, const VT (&xv)[B][NBB], const T (&yv)[RPL]) {
-      T sr[B];
+      T pa[B];  // the lane's partial dot product of each row
 #pragma unroll
       for (int u = 0; u < B; ++u) {
         T a = T(0);
@@ -696,15 +696,23 @@ struct Target {
         for (int b = 0; b < NBB; ++b)
 #pragma unroll
           for (int v = 0; v < V; ++v) a += xv[u][b][v] * bs[b][v];
-        sr[u] = row_sum(a);
+        pa[u] = a;
       }
       T rr[RPL];
 #pragma unroll
       for (int k = 0; k < RPL; ++k) {
-        T sq[B < 4 ? B : 4];
+        // the logit of the row this lane evaluates (row 4k + (lane & 3),
+        // or lane & 1 for two rows per phase)
+        T s;
+        if constexpr (B >= 4) {
+          s = row_sum_pick4(pa[4 * k], pa[4 * k + 1], pa[4 * k + 2],
+                            pa[4 * k + 3], (lane & 1) != 0, (lane & 2) != 0);
+        } else {
+          T sq[B];
 #pragma unroll
-        for (int i = 0; i < (B < 4 ? B : 4); ++i) sq[i] = sr[(B < 4 ? 0 : 4 * k) + i];
-        const T s = pick(sq, B < 4 ? (lane & 1) : (lane & 3));
+          for (int i = 0; i < B; ++i) sq[i] = row_sum(pa[i]);
+          s = pick(sq, lane & 1);
+        }
         const bool valid = r0 + 4 * own(k) + ph < n;
         const bool add = valid && counts;
         T r;
