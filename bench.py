@@ -72,6 +72,9 @@ def parse():
   ap.add_argument('--c3-dual-averaging', action='store_true',
                   help='ragged GLMs: step-size adaptation only (no mass '
                        'matrix)')
+  ap.add_argument('--c3-group-adaptation', action='store_true',
+                  help='ragged GLMs: chain groups in the adaptation steps '
+                       'too')
   ap.add_argument('--no-c3-compare', dest='c3_compare', action='store_false',
                   help='ragged GLMs: skip the dual-averaging-only comparison '
                        'run')

@@ -339,7 +339,8 @@ def _init_steps(specs, adaptation):
   return 0.05
 
 
-def _ragged_run(pa, specs, bij, seed, adaptation, na, nr, tap):
+def _ragged_run(pa, specs, bij, seed, adaptation, na, nr, tap,
+                group_adaptation=False):
   from probability_amd.mcmc import batched
   return batched.RaggedRun(
       specs, 32, nr, bijectors=bij, slot_dim=192, seed=seed,
@@ -347,7 +348,8 @@ def _ragged_run(pa, specs, bij, seed, adaptation, na, nr, tap):
       step_size=_init_steps(specs, adaptation),
       target_accept_prob=tap, max_tree_depth=10, max_iters_per_launch=100,
       adaptation=adaptation, stats={'time_kernels': False},
-      chains_per_block=C3_CHAINS_PER_BLOCK)
+      chains_per_block=C3_CHAINS_PER_BLOCK,
+      group_adaptation=group_adaptation)
 
 
 def _ragged_gate(torch, pa, draws):
@@ -382,7 +384,8 @@ def ragged_glms_512(args, torch, pa):
   specs, bij = _ragged_specs(torch, pa, p)
   _say(f'ragged: {adaptation} adaptation, {per_step} transitions per step')
   if args.warmup:
-    wr = _ragged_run(pa, specs, bij, args.seed + 999, adaptation, na, nr, tap)
+    wr = _ragged_run(pa, specs, bij, args.seed + 999, adaptation, na, nr, tap,
+                     args.c3_group_adaptation)
     wr.advance_to(min(total, args.warmup * per_step))
     torch.cuda.synchronize()
     del wr
@@ -392,7 +395,8 @@ def ragged_glms_512(args, torch, pa):
   for s in range(args.steps):
     if not runs or runs[-1][1] >= total:
       runs.append([_ragged_run(pa, specs, bij, args.seed + len(runs),
-                               adaptation, na, nr, tap), 0])
+                               adaptation, na, nr, tap,
+                               args.c3_group_adaptation), 0])
     run = runs[-1]
     run[1] = min(total, run[1] + per_step)
     run[0].advance_to(run[1])
@@ -490,11 +494,11 @@ def _ragged_roofline(specs, stats, run_s):
   its problem's logits and gradient in one pass over X, two FMAs (4 FLOP)
   per design-matrix entry, summed over the run's leapfrogs, over the run's
   wall time (the size classes' kernels overlap on their streams and cover
-  nearly all of it), against the FP32 peak.  The kernel is compute-bound:
-  timed alone, chain groups that read X once for four chains run at the
-  same rate per chain-leapfrog as one chain per block.  The design-matrix
-  bytes those leapfrogs read (N_k D_k 4 B each, ~95% from L2) are reported
-  beside it, with the HBM traffic from counters."""
+  nearly all of it), against the FP32 peak.  The adaptation steps run one
+  chain per block on FP32 VALU passes; the sampling launches of the 8- and
+  4-wave classes run groups of four chains on v_mfma_f32_4x4x1_16b_f32.
+  The design-matrix bytes those leapfrogs read (N_k D_k 4 B each, ~95% from
+  L2) are reported beside it, with the HBM traffic from counters."""
   lpp = stats['leapfrogs_per_problem']
   entries = float(sum(float(lpp[i]) * s.n_obs * s.dim
                       for i, s in enumerate(specs)))
@@ -504,11 +508,13 @@ def _ragged_roofline(specs, stats, run_s):
          'kernel': 'nuts_kernel<float,3,glm> (all size classes)',
          'what': 'GLM logit and gradient FLOP (4 per design-matrix entry '
                  'per chain-leapfrog) over the run\'s wall time',
-         'limiter': 'FP32 VALU work of the gradient passes (FMAs, DPP row '
-                    'sums, the sigmoid) with no MFMA and packed FP32 off; '
-                    'timed alone at 8192 x 128 with 2048 chains: 0.17 us '
-                    'per chain-leapfrog, 24 TFLOP/s, with or without chain '
-                    'groups',
+         'limiter': 'adaptation steps (one chain per block): FP32 VALU '
+                    'work of the gradient passes (FMAs, DPP row sums, the '
+                    'sigmoid; packed FP32 off); sampling: chain groups of 4 '
+                    'on FP32 MFMA plus in-quad transposes; timed alone at '
+                    '8192 x 128 with 2048 chains: 0.17 us per '
+                    'chain-leapfrog one chain per block (24 TFLOP/s), '
+                    '0.11 us in groups (37 TFLOP/s)',
          'design_matrix_GBps': entries * 4.0 / run_s / 1e9,
          'chain_leapfrogs': float(lpp.sum()),
          'design_matrix_entries': entries,

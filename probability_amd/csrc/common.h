@@ -239,6 +239,46 @@ __device__ inline double dpp_add_from(double x, double src) {
 constexpr int kDppRowShl4 = 0x104;   // lane i reads lane i + 4 (in its row)
 constexpr int kDppRowShr4 = 0x114;   // lane i reads lane i - 4
 constexpr int kDppRowRor8 = 0x128;   // lane i reads lane i ^ 8 (halves swap)
+constexpr int kDppRowRor4 = 0x124;   // lane i reads lane i - 4 (mod 16)
+constexpr int kDppRowRor12 = 0x12C;  // lane i reads lane i + 4 (mod 16)
+
+// src of the lane a DPP pattern names; every lane reads a lane of its own
+// row, so no old operand is needed.
+template <int Ctrl>
+__device__ inline float dpp_read(float src) {
+  return __int_as_float(
+      __builtin_amdgcn_mov_dpp(__float_as_int(src), Ctrl, 0xF, 0xF, false));
+}
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// 4x4 transpose inside each quad of lanes: element t of lane q becomes
+// element q of lane t.  Two exchange stages: lanes q and q^1 swap the
+// elements whose index differs from the lane's in bit 0, then lanes q and
+// q^2 those that differ in bit 1.  b0, b1: bits 0 and 1 of the lane.
+__device__ inline void quad_transpose(f32x4& v, bool b0, bool b1) {
+  const float p0 = dpp_read<kDppQuadXor1>(v[1]);
+  const float p1 = dpp_read<kDppQuadXor1>(v[0]);
+  const float p2 = dpp_read<kDppQuadXor1>(v[3]);
+  const float p3 = dpp_read<kDppQuadXor1>(v[2]);
+  const float y0 = b0 ? p0 : v[0], y1 = b0 ? v[1] : p1;
+  const float y2 = b0 ? p2 : v[2], y3 = b0 ? v[3] : p3;
+  const float s0 = dpp_read<kDppQuadXor2>(y2);
+  const float s1 = dpp_read<kDppQuadXor2>(y3);
+  const float s2 = dpp_read<kDppQuadXor2>(y0);
+  const float s3 = dpp_read<kDppQuadXor2>(y1);
+  v[0] = b1 ? s0 : y0;
+  v[1] = b1 ? s1 : y1;
+  v[2] = b1 ? y2 : s2;
+  v[3] = b1 ? y3 : s3;
+}
+
+// v_mfma_f32_4x4x1_16b_f32: 16 blocks of 4 lanes; block b adds the outer
+// product of its lanes' a (rows) and b (columns) to a 4x4 tile held as
+// register m of lane 4b + n (row m, column n).
+__device__ inline f32x4 mfma_4x4(float a, float b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_4x4x1f32(a, b, c, 0, 0, 0);
+}
 
 // The row sums of four values a0..a3 (the partial dot products of four rows
 // of a batch), delivered so that lane L holds the sum over its 16-lane DPP

@@ -1036,6 +1036,173 @@ struct Target {
     }
   }
 
+  // Float32 groups of 4 chains with at most kMfmaSlabs * 16 coefficients
+  // take their passes on matrix instructions (group_mfma).
+  static constexpr bool kMfmaGroup = sizeof(T) == 4 && G == 4;
+  static constexpr int kMfmaSlabs = 4 * K < 8 ? 4 * K : 8;
+
+  // The pass of a float32 group of 4 chains on v_mfma_f32_4x4x1_16b_f32
+  // (block b: lanes 4b..4b+3; register m of lane 4b + n holds row m, column
+  // n of the block's tile).  Lane L = 16 rg + 4 kp + q; the wave takes
+  // sub-chunks of 16 rows r0 = 16 (grp_w + i grp_waves); NS = live slabs of
+  // 16 coefficients.
+  //  loads: the lane holds row 4 rg + (q + kp) % 4, coefficient quads
+  //   4s + kp (s < NS), one 16-byte load each (64 contiguous bytes of a row
+  //   over the four lanes with equal rg and row);
+  //  logits: MFMA (s, t) with A = x (rows), B = beta of chain n at
+  //   coefficient 16s + 4kp + t: block (rg, kp) sums the quads of phase kp
+  //   of rows 4 rg + (m + kp) % 4; lane (rg, kp, q) adds the four phases of
+  //   row 4 rg + kp, chain q (phase kp + d sits in register (4 - d) % 4 of
+  //   the lane 4d further on, a DPP rotation of the 16-lane row);
+  //  residual: row 4 rg + kp, chain q, evaluated once per (row, chain);
+  //  gradient: each quad's 4x4 tile is transposed across the block, so that
+  //   element t of lane m' is coefficient 16s + 4kp + m' of row
+  //   4 rg + (t + kp) % 4, and MFMA (s, t) multiplies it by that row's
+  //   residual (the residual of the lane 4t further on): register m' of
+  //   acc[s] in lane (rg, kp, q) += x[row, 16s + 4kp + m'] r[row, q].
+  // Rows past n read row n - 1 and get residual zero; quads past the row
+  // pitch read quad 0 and meet beta = 0 (their gradient is never read).
+  // At the end the four row groups rg are added (lanes 16 apart) and this
+  // wave's partial sums go to its slots, as in group_round.
+  template <int NS, bool LINEAR>
+  __device__ void group_mfma(int dx, int xp) {
+    static_assert(sizeof(T) == 4 && G == 4, "float32 groups of 4 chains");
+    static_assert(NS >= 1 && 16 * NS <= K * 64, "slabs within the state");
+    const int lane = lane_id();
+    const int q = lane & 3, kp = (lane >> 2) & 3, rg = lane >> 4;
+    const bool b0 = (q & 1) != 0, b1 = (q & 2) != 0;
+    const int n = n_obs;
+    lptr<const T> beta = (lptr<const T>)(grp + GroupLayout<T, K>::beta(G));
+    lptr<const T> sg = (lptr<const T>)(grp + GroupLayout<T, K>::sig(G));
+    f32x4 bq[NS];
+    int cs[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      const int c = 16 * s + 4 * kp;
+      bq[s] = *(lptr<const f32x4>)(beta + q * K * 64 + c);
+      cs[s] = c < xp ? c : 0;
+    }
+    T inv_s2 = T(1);
+    if (LINEAR) {
+      const T sj = sg[q];
+      inv_s2 = T(1) / (sj * sj);
+    }
+    const int ro = 4 * rg + ((q + kp) & 3);  // row of the lane's loads
+    const int rr = 4 * rg + kp;              // row of the lane's residual
+    double lpl = 0.0;
+    T ssq = T(0);
+    f32x4 acc[NS];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) acc[s] = f32x4{0.f, 0.f, 0.f, 0.f};
+    auto load = [&](int r0, f32x4 (&xv)[NS], T& yv) {
+      if (r0 + 16 <= n) {
+        const gptr<T> xr = x + (size_t)(r0 + ro) * xp;
+#pragma unroll
+        for (int s = 0; s < NS; ++s) xv[s] = *(gptr<f32x4>)(xr + cs[s]);
+        yv = y[r0 + rr];
+        return;
+      }
+      const int j = r0 + ro;
+      const gptr<T> xr = x + (size_t)(j < n ? j : n - 1) * xp;
+#pragma unroll
+      for (int s = 0; s < NS; ++s) xv[s] = *(gptr<f32x4>)(xr + cs[s]);
+      const int jr = r0 + rr;
+      yv = y[jr < n ? jr : n - 1];
+    };
+    auto process = [&](int r0, f32x4 (&xv)[NS], T yv) {
+      f32x4 c = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int s = 0; s < NS; ++s)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) c = mfma_4x4(xv[s][t], bq[s][t], c);
+      // the transposes need only the loaded tiles: they overlap the MFMAs
+#pragma unroll
+      for (int s = 0; s < NS; ++s) quad_transpose(xv[s], b0, b1);
+      T sl = c[0];
+      sl = sl + dpp_read<kDppRowRor12>(c[3]);
+      sl = sl + dpp_read<kDppRowRor8>(c[2]);
+      sl = sl + dpp_read<kDppRowRor4>(c[1]);
+      const bool valid = r0 + rr < n;
+      T r;
+      if (LINEAR) {
+        const T res = yv - sl;
+        r = res * inv_s2;
+        lpl += valid ? (double)(T(-0.5) * res * res * inv_s2) : 0.0;
+        ssq += valid ? res * res : T(0);
+      } else {
+        const T w0 = T(1) - yv;
+        const SoftplusParts<T> sp = softplus_parts(sl);
+        const T l0 = w0 == T(0) ? T(0) : -(fmax(sl, T(0)) + sp.l) * w0;
+        const T l1 = yv == T(0) ? T(0) : -(fmax(-sl, T(0)) + sp.l) * yv;
+        lpl += valid ? (double)(l0 + l1) : 0.0;
+        r = yv - sp.sig;
+      }
+      r = valid ? r : T(0);
+      const T rt1 = dpp_read<kDppRowRor12>(r);
+      const T rt2 = dpp_read<kDppRowRor8>(r);
+      const T rt3 = dpp_read<kDppRowRor4>(r);
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        acc[s] = mfma_4x4(xv[s][0], r, acc[s]);
+        acc[s] = mfma_4x4(xv[s][1], rt1, acc[s]);
+        acc[s] = mfma_4x4(xv[s][2], rt2, acc[s]);
+        acc[s] = mfma_4x4(xv[s][3], rt3, acc[s]);
+      }
+    };
+    int r0 = 16 * grp_w;
+    const int rs = 16 * grp_waves;
+    if (r0 < n) {
+      // two register buffers used in turn, unrolled by two (no copies)
+      f32x4 xa[NS], xb[NS];
+      T ya, yb;
+      load(r0, xa, ya);
+      for (;;) {
+        const int r1 = r0 + rs;
+        const bool m1 = r1 < n;
+        if (m1) load(r1, xb, yb);
+        process(r0, xa, ya);
+        if (!m1) break;
+        const int r2 = r1 + rs;
+        const bool m2 = r2 < n;
+        if (m2) load(r2, xa, ya);
+        process(r1, xb, yb);
+        if (!m2) break;
+        r0 = r2;
+      }
+    }
+    // add the row groups (lanes 16 and 32 apart) and leave this wave's
+    // partial sums in its slots; slabs past NS are zero
+    lptr<T> red_g = (lptr<T>)(grp + GroupLayout<T, K>::red(G));
+    lptr<double> red_l =
+        (lptr<double>)(grp + GroupLayout<T, K>::lp(G, grp_waves));
+    lptr<T> red_s = (lptr<T>)(grp + GroupLayout<T, K>::ssq(G, grp_waves));
+    lptr<T> dst = red_g + ((size_t)grp_w * G + q) * K * 64;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      f32x4 v = acc[s];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const T s1 = v[e] + __shfl_xor(v[e], 16);
+        v[e] = s1 + __shfl_xor(s1, 32);
+      }
+      if (lane < 16) *(lptr<f32x4>)(dst + 16 * s + 4 * kp) = v;
+    }
+#pragma unroll
+    for (int s = NS; s < 4 * K; ++s)
+      if (lane < 16)
+        *(lptr<f32x4>)(dst + 16 * s + 4 * kp) = f32x4{0.f, 0.f, 0.f, 0.f};
+    // log-likelihood (and residual) sums over the lanes of each chain q
+#pragma unroll
+    for (int m = 4; m < 64; m <<= 1) {
+      lpl += __shfl_xor(lpl, m);
+      if (LINEAR) ssq += __shfl_xor(ssq, m);
+    }
+    if (lane < 4) {
+      red_l[grp_w * G + q] = lpl;
+      red_s[grp_w * G + q] = LINEAR ? ssq : T(0);
+    }
+  }
+
   // One round: every wave passes over its rows for all G chains and leaves
   // its partial sums in LDS.  Returns false (after the first barrier) once
   // every owner has set its done flag.
@@ -1054,6 +1221,23 @@ struct Target {
     const int lane = lane_id();
     const int dx = LINEAR ? dim - 1 : dim;
     const int xp = x_pitch;
+    if constexpr (kMfmaGroup) {
+      const int ns = (dx + 15) >> 4;  // live slabs of 16 coefficients
+      if (ns <= kMfmaSlabs) {
+        switch (ns) {
+          case 1: group_mfma<1, LINEAR>(dx, xp); break;
+          case 2: group_mfma<(kMfmaSlabs >= 2 ? 2 : 1), LINEAR>(dx, xp); break;
+          case 3: group_mfma<(kMfmaSlabs >= 3 ? 3 : 1), LINEAR>(dx, xp); break;
+          case 4: group_mfma<(kMfmaSlabs >= 4 ? 4 : 1), LINEAR>(dx, xp); break;
+          case 5: group_mfma<(kMfmaSlabs >= 5 ? 5 : 1), LINEAR>(dx, xp); break;
+          case 6: group_mfma<(kMfmaSlabs >= 6 ? 6 : 1), LINEAR>(dx, xp); break;
+          case 7: group_mfma<(kMfmaSlabs >= 7 ? 7 : 1), LINEAR>(dx, xp); break;
+          default: group_mfma<kMfmaSlabs, LINEAR>(dx, xp); break;
+        }
+        group_barrier();  // 2: every wave's partial sums are in LDS
+        return true;
+      }
+    }
     const int grp16 = lane & 15;
     int cst[NB];
     VT acc[G][NB];

@@ -64,10 +64,15 @@ class PosteriorBatch:
       multiple of it (else 1 is used).  Results depend on it only through
       the order of floating-point sums; equal values give bit-identical
       results alone or in any mix.
+    group_adaptation: use the chain groups in single-transition launches
+      (adaptation steps) too; by default only multi-transition launches
+      use them, since a single transition of a group lasts as long as its
+      longest trajectory.
   """
 
   def __init__(self, specs, num_chains, bijectors=None, keys=None,
-               slot_dim=None, waves_per_chain=None, chains_per_block=1):
+               slot_dim=None, waves_per_chain=None, chains_per_block=1,
+               group_adaptation=False):
     if not specs:
       raise ValueError('need at least one problem')
     self.specs = [targets_lib.as_target(s) for s in specs]
@@ -111,6 +116,7 @@ class PosteriorBatch:
     if _family(self.kind) != 1 or any(n % g for n in self.num_chains):
       g = 1
     self.chains_per_block = g
+    self.group_adaptation = bool(group_adaptation)
     if _family(self.kind) != 1:
       self.waves_per_chain = 1
     elif waves_per_chain is not None:
@@ -187,7 +193,7 @@ def sample_posteriors(batch, num_results, num_burnin_steps=0,
 
 def sample_ragged(specs, num_chains, num_results, bijectors=None, keys=None,
                   slot_dim=None, progress=None, stats=None, chains_per_block=1,
-                  **kwargs):
+                  group_adaptation=False, **kwargs):
   """sample_posteriors for problems of very different sizes: the GLMs are
   split into classes by the waves that share one of their chains (from each
   problem's own N x D), every class is its own batch on its own stream, and
@@ -195,12 +201,14 @@ def sample_ragged(specs, num_chains, num_results, bijectors=None, keys=None,
   not wait for the largest ones' adaptation steps).  A problem's draws
   depend only on its own size class, key and chain count, so they equal its
   draws alone.  chains_per_block: an int, or a dict from a class's waves per
-  chain to the chains per block of its sampling launches.  Returns
-  BatchResults in the order of `specs`.
+  chain to the chains per block of its sampling launches (and, with
+  group_adaptation, of its adaptation steps).  Returns BatchResults in the
+  order of `specs`.
   """
   run = RaggedRun(specs, num_chains, num_results, bijectors=bijectors,
                   keys=keys, slot_dim=slot_dim, stats=stats,
-                  chains_per_block=chains_per_block, **kwargs)
+                  chains_per_block=chains_per_block,
+                  group_adaptation=group_adaptation, **kwargs)
   while not run.done():
     run.advance_launches()
     if progress is not None:
@@ -215,7 +223,7 @@ class RaggedRun:
 
   def __init__(self, specs, num_chains, num_results, bijectors=None,
                keys=None, slot_dim=None, stats=None, chains_per_block=1,
-               **kwargs):
+               group_adaptation=False, **kwargs):
     if not specs:
       raise ValueError('need at least one problem')
     specs = [targets_lib.as_target(s) for s in specs]
@@ -251,7 +259,8 @@ class RaggedRun:
                              [chains[i] for i in idx],
                              [bijs[i] for i in idx], [keys[i] for i in idx],
                              slot_dim=slot_dim, waves_per_chain=w,
-                             chains_per_block=g)
+                             chains_per_block=g,
+                             group_adaptation=group_adaptation)
       stream = torch.cuda.Stream(device=dev)
       stream.wait_stream(self.caller)
       sub = {} if stats is not None else None
@@ -577,9 +586,10 @@ class _Run:
     a.unrolled_leapfrog_steps = self.unrolled
     a.target_kind = b.kind
     a.waves_per_chain = b.waves_per_chain
-    # chain groups only for multi-transition launches: a single transition
-    # (an adaptation step) is bound by each group's slowest chain
-    a.chains_per_block = b.chains_per_block if n > 1 else 1
+    # chain groups in multi-transition launches, and in single transitions
+    # (adaptation steps, bound by each group's slowest chain) on request
+    a.chains_per_block = (b.chains_per_block if n > 1 or b.group_adaptation
+                          else 1)
     a.max_energy_diff = float(self.max_energy_diff)
     a.problems = b.table.data_ptr()
     a.n_problems = b.n_problems

@@ -195,3 +195,57 @@ def test_chain_groups_alone_equals_mix():
     alone = _run(mk([specs[k]], [bij[k]], [k]), seed=8)
     np.testing.assert_array_equal(mix.draws[k].cpu().numpy(),
                                   alone.draws[0].cpu().numpy())
+
+
+def _wide_group_problems():
+  # float32 groups of 4 take their passes on matrix instructions for up to
+  # 128 coefficients (8 slabs of 16), state widths of 2 and 3 slots; row
+  # counts that are not multiples of the 16-row sub-chunks
+  r = np.random.default_rng(33)
+  specs, bij, oracles = [], [], []
+  for k, (nobs, d) in enumerate(((1000, 128), (523, 128), (37, 20),
+                                 (611, 113))):
+    x = r.normal(size=(nobs, d)).astype(np.float32) * 0.2
+    beta = r.normal(size=d) * 0.2
+    if k % 2 == 0:
+      y = (r.uniform(size=nobs) < 1 / (1 + np.exp(-(x @ beta)))).astype(
+          np.float32)
+      specs.append(ptg.LogisticRegression(x, y, dtype=torch.float32))
+      bij.append(None)
+      oracles.append(otg.LogisticRegression(x.astype(np.float64),
+                                            y.astype(np.float64)))
+    else:
+      y = (x @ beta + 0.5 * r.normal(size=nobs)).astype(np.float32)
+      specs.append(ptg.LinearRegression(x, y, dtype=torch.float32))
+      bij.append([pbij.Identity(), pbij.Exp()])
+      codes = np.zeros(d + 1, np.int32)
+      codes[d] = 2
+      oracles.append(otg.Transformed(
+          otg.LinearRegression(x.astype(np.float64), y.astype(np.float64)),
+          codes))
+  return specs, bij, oracles
+
+
+@pytest.mark.parametrize('waves', [4, 8])
+def test_wide_chain_groups_match_oracle(waves):
+  specs, bij, oracles = _wide_group_problems()
+  chains = 8
+  batch = batched.PosteriorBatch(specs, chains, bij, slot_dim=192,
+                                 waves_per_chain=waves, chains_per_block=4)
+  assert batch.chains_per_block == 4
+  res = _run(batch, seed=6)
+  for k, ot in enumerate(oracles):
+    d = specs[k].dim
+    ox, otrace, _ = omcmc.sample_nuts_da(
+        ot, np.zeros((chains, d)), np.asarray(0.05, np.float32), 6, 5, 6,
+        seed=orng.sanitize_seed(6), target_accept_prob=0.8, dtype=np.float32,
+        problem=k)
+    ostates = np.stack([ot.forward(v) if hasattr(ot, 'forward') else v
+                        for v in ox])
+    lf = np.stack([r.leapfrogs_taken for r in otrace])
+    np.testing.assert_array_equal(res.leapfrogs_taken[k].cpu().numpy(), lf)
+    got = res.draws[k].double().cpu().numpy()
+    err = np.abs(got - ostates).max() / max(np.abs(ostates).max(), 1.0)
+    print(f'ERR wide group problem {k} N={specs[k].n_obs} D={d} W={waves} '
+          f'draws rel={err:.3e}', flush=True)
+    assert err <= 2e-4, (k, err)
