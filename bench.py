@@ -75,6 +75,9 @@ def parse():
   ap.add_argument('--c3-group-adaptation', action='store_true',
                   help='ragged GLMs: chain groups in the adaptation steps '
                        'too')
+  ap.add_argument('--c3-class-splits', default=None,
+                  help="ragged GLMs: 'w:n,...' deals the w-wave class to n "
+                       'independent batches (default: none)')
   ap.add_argument('--no-c3-compare', dest='c3_compare', action='store_false',
                   help='ragged GLMs: skip the dual-averaging-only comparison '
                        'run')

@@ -328,6 +328,24 @@ C3_HBM_BYTES_PER_ENTRY = (1068320259 * 2.0 + 230789418 * 1.0) * 1024 / 4.878e12
 # sampling launches of the 8- and 4-wave classes run chain groups of 4
 # (one pass over X per gradient for 4 chains of a problem)
 C3_CHAINS_PER_BLOCK = {8: 4, 4: 4}
+# size classes split into independent batches (by --c3-class-splits); none
+# by default: with the runtime's 4 hardware queues per process the extra
+# streams share queues and the run slows (8:4 48 s against 34.6 s unsplit;
+# with 8 queues 8:4 takes 32.2 s), because every adaptation step is set by
+# its longest trajectory, which a split does not shorten
+C3_CLASS_SPLITS = {}
+
+
+def _class_splits(args):
+  """--c3-class-splits 'w:n,...' or the default."""
+  text = getattr(args, 'c3_class_splits', None)
+  if not text:
+    return dict(C3_CLASS_SPLITS)
+  out = {}
+  for part in text.split(','):
+    w, n = part.split(':')
+    out[int(w)] = int(n)
+  return out
 
 
 def _init_steps(specs, adaptation):
@@ -340,7 +358,7 @@ def _init_steps(specs, adaptation):
 
 
 def _ragged_run(pa, specs, bij, seed, adaptation, na, nr, tap,
-                group_adaptation=False):
+                group_adaptation=False, class_splits=None):
   from probability_amd.mcmc import batched
   return batched.RaggedRun(
       specs, 32, nr, bijectors=bij, slot_dim=192, seed=seed,
@@ -349,7 +367,7 @@ def _ragged_run(pa, specs, bij, seed, adaptation, na, nr, tap,
       target_accept_prob=tap, max_tree_depth=10, max_iters_per_launch=100,
       adaptation=adaptation, stats={'time_kernels': False},
       chains_per_block=C3_CHAINS_PER_BLOCK,
-      group_adaptation=group_adaptation)
+      group_adaptation=group_adaptation, class_splits=class_splits)
 
 
 def _ragged_gate(torch, pa, draws):
@@ -385,7 +403,7 @@ def ragged_glms_512(args, torch, pa):
   _say(f'ragged: {adaptation} adaptation, {per_step} transitions per step')
   if args.warmup:
     wr = _ragged_run(pa, specs, bij, args.seed + 999, adaptation, na, nr, tap,
-                     args.c3_group_adaptation)
+                     args.c3_group_adaptation, _class_splits(args))
     wr.advance_to(min(total, args.warmup * per_step))
     torch.cuda.synchronize()
     del wr
@@ -396,7 +414,8 @@ def ragged_glms_512(args, torch, pa):
     if not runs or runs[-1][1] >= total:
       runs.append([_ragged_run(pa, specs, bij, args.seed + len(runs),
                                adaptation, na, nr, tap,
-                               args.c3_group_adaptation), 0])
+                               args.c3_group_adaptation,
+                               _class_splits(args)), 0])
     run = runs[-1]
     run[1] = min(total, run[1] + per_step)
     run[0].advance_to(run[1])
@@ -472,7 +491,8 @@ def ragged_glms_512(args, torch, pa):
   if done_runs:
     line['roofline'] = _ragged_roofline(specs, done_runs[-1][1], per_run_s)
   if args.c3_compare and adaptation == 'windowed':
-    rr = _ragged_run(pa, specs, bij, args.seed, 'dual_averaging', na, nr, 0.8)
+    rr = _ragged_run(pa, specs, bij, args.seed, 'dual_averaging', na, nr, 0.8,
+                     class_splits=_class_splits(args))
     torch.cuda.synchronize()
     ta = time.perf_counter()
     rr.advance_to(total)

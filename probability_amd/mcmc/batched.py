@@ -193,7 +193,7 @@ def sample_posteriors(batch, num_results, num_burnin_steps=0,
 
 def sample_ragged(specs, num_chains, num_results, bijectors=None, keys=None,
                   slot_dim=None, progress=None, stats=None, chains_per_block=1,
-                  group_adaptation=False, **kwargs):
+                  group_adaptation=False, class_splits=None, **kwargs):
   """sample_posteriors for problems of very different sizes: the GLMs are
   split into classes by the waves that share one of their chains (from each
   problem's own N x D), every class is its own batch on its own stream, and
@@ -202,13 +202,18 @@ def sample_ragged(specs, num_chains, num_results, bijectors=None, keys=None,
   depend only on its own size class, key and chain count, so they equal its
   draws alone.  chains_per_block: an int, or a dict from a class's waves per
   chain to the chains per block of its sampling launches (and, with
-  group_adaptation, of its adaptation steps).  Returns BatchResults in the
-  order of `specs`.
+  group_adaptation, of its adaptation steps).  class_splits: a dict from a
+  class's waves per chain to the number of batches (each on its own
+  stream) its problems are dealt to, by cost: every adaptation step of a
+  batch lasts as long as its longest trajectory, so batches that advance
+  independently keep one problem's long trajectories from holding up the
+  others.  Returns BatchResults in the order of `specs`.
   """
   run = RaggedRun(specs, num_chains, num_results, bijectors=bijectors,
                   keys=keys, slot_dim=slot_dim, stats=stats,
                   chains_per_block=chains_per_block,
-                  group_adaptation=group_adaptation, **kwargs)
+                  group_adaptation=group_adaptation,
+                  class_splits=class_splits, **kwargs)
   while not run.done():
     run.advance_launches()
     if progress is not None:
@@ -223,7 +228,7 @@ class RaggedRun:
 
   def __init__(self, specs, num_chains, num_results, bijectors=None,
                keys=None, slot_dim=None, stats=None, chains_per_block=1,
-               group_adaptation=False, **kwargs):
+               group_adaptation=False, class_splits=None, **kwargs):
     if not specs:
       raise ValueError('need at least one problem')
     specs = [targets_lib.as_target(s) for s in specs]
@@ -249,10 +254,15 @@ class RaggedRun:
     dev = specs[0].device
     self.caller = torch.cuda.current_stream(dev)
     self.runs = []
+    cost = [s.n_obs * s.dim if glm else s.dim for s in specs]
+    subs = []
     for w in sorted(set(cls), reverse=True):
-      idx = [i for i in range(p) if cls[i] == w]
+      members = [i for i in range(p) if cls[i] == w]
+      n_sub = int((class_splits or {}).get(w, 1))
+      subs += [(w, idx) for idx in _split_by_cost(members, cost, n_sub)]
+    for w, idx in subs:
       if glm:
-        idx = _xcd_order(idx, [specs[i].n_obs * specs[i].dim for i in idx])
+        idx = _xcd_order(idx, [cost[i] for i in idx])
       g = (chains_per_block.get(w, 1) if isinstance(chains_per_block, dict)
            else chains_per_block)
       batch = PosteriorBatch([specs[i] for i in idx],
@@ -354,6 +364,20 @@ def _init_uniform(batch, seed):
         _lib.stream_handle(batch.device)))
     x[a:b, :d] = out
   return x
+
+
+def _split_by_cost(idx, cost, n):
+  """idx dealt to at most n non-empty lists of similar total cost: most
+  expensive first, each to the list with the least cost so far."""
+  n = max(1, min(int(n), len(idx)))
+  if n == 1:
+    return [list(idx)]
+  bins, load = [[] for _ in range(n)], [0.0] * n
+  for i in sorted(idx, key=lambda i: -cost[i]):
+    k = min(range(n), key=lambda j: load[j])
+    bins[k].append(i)
+    load[k] += cost[i]
+  return bins
 
 
 def _xcd_order(idx, cost, n_xcd=8):

@@ -37,3 +37,22 @@ def test_xcd_order_is_a_balanced_permutation(n):
 def test_sample_ragged_rejects_no_problems():
   with pytest.raises(ValueError):
     batched.sample_ragged([], 4, num_results=1)
+
+
+@pytest.mark.parametrize('n,k', [(1, 4), (3, 4), (10, 1), (192, 4), (50, 8)])
+def test_split_by_cost_partitions_with_balanced_cost(n, k):
+  r = np.random.default_rng(n + k)
+  idx = list(range(7, 7 + n))
+  cost = {i: int(c) for i, c in zip(idx, r.integers(1, 10**5, size=n))}
+  full = [0] * (7 + n)
+  for i, c in cost.items():
+    full[i] = c
+  parts = batched._split_by_cost(idx, full, k)
+  # every problem once, no empty part, at most k parts
+  assert sorted(i for p in parts for i in p) == idx
+  assert len(parts) == min(k, n) and all(parts)
+  if len(parts) > 1:
+    # greedy by decreasing cost: no part exceeds another by more than the
+    # largest single cost
+    totals = [sum(cost[i] for i in p) for p in parts]
+    assert max(totals) - min(totals) <= max(cost.values())
