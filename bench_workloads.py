@@ -528,13 +528,13 @@ def _ragged_roofline(specs, stats, run_s):
          'kernel': 'nuts_kernel<float,3,glm> (all size classes)',
          'what': 'GLM logit and gradient FLOP (4 per design-matrix entry '
                  'per chain-leapfrog) over the run\'s wall time',
-         'limiter': 'adaptation steps (one chain per block): FP32 VALU '
-                    'work of the gradient passes (FMAs, DPP row sums, the '
-                    'sigmoid; packed FP32 off); sampling: chain groups of 4 '
-                    'on FP32 MFMA plus in-quad transposes; timed alone at '
-                    '8192 x 128 with 2048 chains: 0.17 us per '
-                    'chain-leapfrog one chain per block (24 TFLOP/s), '
-                    '0.11 us in groups (37 TFLOP/s)',
+         'limiter': 'adaptation steps of the 8-wave class: each lasts '
+                    'as long as its longest trajectory at one CU\'s rate '
+                    '(one chain per block); sampling launches: chain '
+                    'groups of 4 on FP32 MFMA, bound by fetching X from L2 '
+                    '(~15 B/clk per CU); timed alone at 8192 x 128 with '
+                    '2048 chains: 0.17 us per chain-leapfrog one chain per '
+                    'block, 0.11 us in groups',
          'design_matrix_GBps': entries * 4.0 / run_s / 1e9,
          'chain_leapfrogs': float(lpp.sum()),
          'design_matrix_entries': entries,
