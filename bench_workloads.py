@@ -316,13 +316,13 @@ def _ragged_specs(torch, pa, n_problems):
 
 
 C3_STEP_TRANSITIONS = 75     # one bench step: 1/20 of a 1500-transition run
-# HBM bytes the GLM NUTS kernel moves per design-matrix entry streamed:
-# rocprofv3 FETCH_SIZE (1 068 320 259 KiB) x 2.00 + WRITE_SIZE
-# (230 789 418 KiB) x 1.00 over a shortened run of this workload that
-# streamed 4.878e12 entries, the factors being what the counters read for a
-# 256 MiB device copy (profiles/nuts_kernel_glm_ragged512.md); most X reads
-# hit L2.
-C3_HBM_BYTES_PER_ENTRY = (1068320259 * 2.0 + 230789418 * 1.0) * 1024 / 4.878e12
+# HBM bytes the GLM NUTS kernels (one chain per block and the chain
+# groups) move per design-matrix entry streamed: rocprofv3 FETCH_SIZE
+# (885 980 000 KiB) x 2.00 + WRITE_SIZE (172 120 300 KiB) x 1.00 over a
+# shortened run of this workload that streamed 4.763e12 entries, the
+# factors being what the counters read for a 256 MiB device copy
+# (profiles/nuts_kernel_glm_ragged512.md); most X reads hit L2.
+C3_HBM_BYTES_PER_ENTRY = (885980000 * 2.0 + 172120300 * 1.0) * 1024 / 4.763e12
 
 
 # sampling launches of the 8- and 4-wave classes run chain groups of 4

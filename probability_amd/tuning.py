@@ -86,9 +86,10 @@ def _problem(kind, n, d, seed, dtype):
   return targets.LinearRegression(x, y, dtype=dtype)
 
 
-def _time_launch(spec, chains, waves, sampler, n_iters=2):
+def _time_launch(spec, chains, waves, sampler, n_iters=2, chains_per_block=1):
   """Microseconds per chain-leapfrog of one sampling launch with `waves`
-  waves per chain (median of 3 after a warm-up)."""
+  waves per chain (NUTS GLMs: and chain groups of `chains_per_block`;
+  median of 3 after a warm-up)."""
   from probability_amd.mcmc import _runner
   dev, dt = spec.device, spec.dtype
   x = torch.zeros((chains, spec.dim), dtype=dt, device=dev)
@@ -110,6 +111,7 @@ def _time_launch(spec, chains, waves, sampler, n_iters=2):
                       step, [1, rep], n_iters=n_iters, fold_seed=True)
     a = run.args
     a.waves_per_chain = int(waves)
+    a.chains_per_block = int(chains_per_block)
     if sampler == 'nuts':
       a.max_tree_depth, a.unrolled_leapfrog_steps = 5, 1
       a.max_energy_diff = 1000.0
@@ -166,8 +168,25 @@ def main():
   ap = argparse.ArgumentParser()
   ap.add_argument('--quick', action='store_true')
   ap.add_argument('--out', default=TABLE_PATH)
+  ap.add_argument('--time', nargs=5, default=None,
+                  metavar=('KIND', 'N', 'D', 'CHAINS', 'WAVES'),
+                  help='time one configuration (logistic or linear, N x D, '
+                       'chains, waves per chain) and write no table')
+  ap.add_argument('--chains-per-block', type=int, default=1,
+                  help='with --time: NUTS chain groups of this size')
+  ap.add_argument('--sampler', default='nuts', choices=['nuts', 'hmc'])
   args = ap.parse_args()
   _lib.require_cuda()
+  if args.time:
+    kind, n, d, c, w = args.time
+    spec = _problem(kind, int(n), int(d), 1, torch.float32)
+    us = _time_launch(spec, int(c), int(w), args.sampler,
+                      chains_per_block=args.chains_per_block)
+    print(json.dumps({'sampler': args.sampler, 'kind': kind, 'n_obs': int(n),
+                      'dim': spec.dim, 'chains': int(c), 'waves': int(w),
+                      'chains_per_block': args.chains_per_block,
+                      'us_per_chain_leapfrog': us}), flush=True)
+    return
   rows = tune(quick=args.quick, log=lambda s: print(s, flush=True))
   table = {'device': torch.cuda.get_device_name(0),
            'what': 'microseconds per chain-leapfrog of the sampling kernels '
