@@ -1221,6 +1221,26 @@ struct Target {
     const int lane = lane_id();
     const int dx = LINEAR ? dim - 1 : dim;
     const int xp = x_pitch;
+    // a design the passes cannot read (as in glm: rows not 16-byte aligned,
+    // a pitch below the coefficients, an empty X) gives NaN partial sums
+    if ((xp & 3) != 0 || xp < dx || dx < 1 || n_obs < 1) {
+      lptr<T> red_g = (lptr<T>)(grp + GroupLayout<T, K>::red(G));
+      lptr<double> red_l =
+          (lptr<double>)(grp + GroupLayout<T, K>::lp(G, grp_waves));
+      lptr<T> red_s = (lptr<T>)(grp + GroupLayout<T, K>::ssq(G, grp_waves));
+#pragma unroll
+      for (int j = 0; j < G; ++j) {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+          red_g[((size_t)grp_w * G + j) * K * 64 + k * 64 + lane] = T(NAN);
+        if (lane == 0) {
+          red_l[grp_w * G + j] = (double)NAN;
+          red_s[grp_w * G + j] = T(NAN);
+        }
+      }
+      group_barrier();  // 2
+      return true;
+    }
     if constexpr (kMfmaGroup) {
       const int ns = (dx + 15) >> 4;  // live slabs of 16 coefficients
       if (ns <= kMfmaSlabs) {
