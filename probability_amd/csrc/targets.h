@@ -687,7 +687,21 @@ struct Target {
       }
     };
     // one batch: logits, residuals and log-likelihood terms, gradient
-    auto process = [&](int r0, const VT (&xv)[B][NBB], const T (&yv)[RPL]) {
+    // one batch; `next` issues the loads of the following batch.  Every
+    // register of this batch is waited for (the empty asm statements use
+    // them) before those loads are issued: across the loop's back edge the
+    // wait-count pass does not tell the two buffers' loads apart and would
+    // otherwise wait for the newer batch's loads as well, exposing their
+    // whole latency every second batch.
+    auto process = [&](int r0, const VT (&xv)[B][NBB], const T (&yv)[RPL],
+                       auto&& next) {
+#pragma unroll
+      for (int u = 0; u < B; ++u)
+#pragma unroll
+        for (int b = 0; b < NBB; ++b) asm volatile("" ::"v"(xv[u][b]));
+#pragma unroll
+      for (int k = 0; k < RPL; ++k) asm volatile("" ::"v"(yv[k]));
+      next();
       T pa[B];  // the lane's partial dot product of each row
 #pragma unroll
       for (int u = 0; u < B; ++u) {
@@ -750,13 +764,15 @@ struct Target {
     for (int q = 1;; q += 2) {
       const int r1 = row0(q);
       const bool m1 = r1 < n;
-      if (m1) load(r1, xb, yb);
-      process(r0, xa, ya);
+      process(r0, xa, ya, [&] {
+        if (m1) load(r1, xb, yb);
+      });
       if (!m1) break;
       const int r2 = row0(q + 1);
       const bool m2 = r2 < n;
-      if (m2) load(r2, xa, ya);
-      process(r1, xb, yb);
+      process(r1, xb, yb, [&] {
+        if (m2) load(r2, xa, ya);
+      });
       if (!m2) break;
       r0 = r2;
     }
