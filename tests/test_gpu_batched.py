@@ -132,6 +132,29 @@ def test_sample_ragged_equals_each_problem_alone(dtype):
     assert float(mix.step_size[k]) == float(alone.step_size[0])
 
 
+def test_sample_ragged_class_splits_equal_unsplit():
+  # a size class dealt by cost to several batches (class_splits), each on
+  # its own stream and advancing at its own pace: every problem's draws,
+  # trees and tuned step size equal those of the class's single batch
+  specs, bij = _ragged_problems(torch.float32)
+  specs, bij = specs * 2, bij * 2
+  chains = [4, 6, 5, 3, 7, 3] * 2
+  kw = dict(num_results=4, num_burnin_steps=5, num_adaptation_steps=5,
+            step_size=0.05, target_accept_prob=0.8, seed=9, slot_dim=64)
+  one, split = {}, {}
+  base = batched.sample_ragged(specs, chains, bijectors=bij, stats=one, **kw)
+  splits = {w: 3 for _, w in one['classes']}
+  res = batched.sample_ragged(specs, chains, bijectors=bij, stats=split,
+                              class_splits=splits, **kw)
+  assert len(split['classes']) > len(one['classes'])
+  assert sum(n for n, _ in split['classes']) == len(specs)
+  for k in range(len(specs)):
+    for field in ('draws', 'leapfrogs_taken', 'final_state'):
+      np.testing.assert_array_equal(getattr(res, field)[k].cpu().numpy(),
+                                    getattr(base, field)[k].cpu().numpy())
+    assert float(res.step_size[k]) == float(base.step_size[k])
+
+
 def _glm_group_problems(dtype):
   r = np.random.default_rng(21)
   specs, bij, oracles = [], [], []
@@ -177,6 +200,32 @@ def test_chain_groups_match_oracle(group, waves, dtype, ndt, tol):
     ox, otrace, _ = omcmc.sample_nuts_da(
         ot, np.zeros((chains, d)), np.asarray(0.05, ndt), 6, 5, 6,
         seed=orng.sanitize_seed(4), target_accept_prob=0.8, dtype=ndt,
+        problem=k)
+    ostates = np.stack([ot.forward(v) if hasattr(ot, 'forward') else v
+                        for v in ox])
+    lf = np.stack([r.leapfrogs_taken for r in otrace])
+    np.testing.assert_array_equal(res.leapfrogs_taken[k].cpu().numpy(), lf)
+    np.testing.assert_allclose(res.draws[k].cpu().numpy(), ostates, rtol=tol,
+                               atol=tol)
+
+
+@pytest.mark.parametrize('dtype,ndt,tol', [(torch.float32, np.float32, 2e-4),
+                                           (torch.float64, np.float64, 1e-9)])
+def test_group_adaptation_matches_oracle(dtype, ndt, tol):
+  # chain groups of 4 also in the single-transition adaptation launches
+  # (group_adaptation): each chain still follows the oracle's transitions
+  specs, bij, oracles = _glm_group_problems(dtype)
+  chains = 8
+  batch = batched.PosteriorBatch(specs, chains, bij, slot_dim=128,
+                                 waves_per_chain=8, chains_per_block=4,
+                                 group_adaptation=True)
+  assert batch.group_adaptation
+  res = _run(batch, seed=12)
+  for k, ot in enumerate(oracles):
+    d = specs[k].dim
+    ox, otrace, _ = omcmc.sample_nuts_da(
+        ot, np.zeros((chains, d)), np.asarray(0.05, ndt), 6, 5, 6,
+        seed=orng.sanitize_seed(12), target_accept_prob=0.8, dtype=ndt,
         problem=k)
     ostates = np.stack([ot.forward(v) if hasattr(ot, 'forward') else v
                         for v in ox])
