@@ -316,13 +316,17 @@ def _ragged_specs(torch, pa, n_problems):
 
 
 C3_STEP_TRANSITIONS = 75     # one bench step: 1/20 of a 1500-transition run
+# transitions per sampling launch: the whole sampling phase in one launch per
+# size class (each launch ends with a tail in which its last groups finish
+# alone; 100 per launch took 33.7 s per run against 31.1 s for 1000)
+C3_ITERS_PER_LAUNCH = 1000
 # HBM bytes the GLM NUTS kernels (one chain per block and the chain
 # groups) move per design-matrix entry streamed: rocprofv3 FETCH_SIZE
-# (885 980 000 KiB) x 2.00 + WRITE_SIZE (172 120 300 KiB) x 1.00 over a
+# (907 401 286 KiB) x 2.00 + WRITE_SIZE (169 728 112 KiB) x 1.00 over a
 # shortened run of this workload that streamed 4.763e12 entries, the
 # factors being what the counters read for a 256 MiB device copy
 # (profiles/nuts_kernel_glm_ragged512.md); most X reads hit L2.
-C3_HBM_BYTES_PER_ENTRY = (885980000 * 2.0 + 172120300 * 1.0) * 1024 / 4.763e12
+C3_HBM_BYTES_PER_ENTRY = (907401286 * 2.0 + 169728112 * 1.0) * 1024 / 4.763e12
 
 
 # sampling launches of the 8- and 4-wave classes run chain groups of 4
@@ -364,7 +368,8 @@ def _ragged_run(pa, specs, bij, seed, adaptation, na, nr, tap,
       specs, 32, nr, bijectors=bij, slot_dim=192, seed=seed,
       num_burnin_steps=na, num_adaptation_steps=na,
       step_size=_init_steps(specs, adaptation),
-      target_accept_prob=tap, max_tree_depth=10, max_iters_per_launch=100,
+      target_accept_prob=tap, max_tree_depth=10,
+      max_iters_per_launch=C3_ITERS_PER_LAUNCH,
       adaptation=adaptation, stats={'time_kernels': False},
       chains_per_block=C3_CHAINS_PER_BLOCK,
       group_adaptation=group_adaptation, class_splits=class_splits)
@@ -386,9 +391,11 @@ def ragged_glms_512(args, torch, pa):
   C3_STEP_TRANSITIONS transitions of all 512 problems, so 20 steps are one
   complete run (500 adaptation + 1000 draws); the timed region holds exactly
   `steps` steps: complete runs with seeds seed, seed + 1, ... (a last partial
-  run is timed but its draws are not counted).  Warm-up steps run a separate
-  run's first transitions.  value = sum of min-ESS over the problems at
-  split R-hat < 1.01 (failing problems count zero) / timed seconds."""
+  run is timed but its draws are not counted), each enqueued up to its last
+  timed transition at once, so that launches are not cut at the steps'
+  boundaries.  Warm-up steps run a separate run's first transitions.
+  value = sum of min-ESS over the problems at split R-hat < 1.01 (failing
+  problems count zero) / timed seconds."""
   from probability_amd import _lib
   from probability_amd.mcmc import batched
   p, chains = 512, 32
@@ -407,23 +414,24 @@ def ragged_glms_512(args, torch, pa):
     wr.advance_to(min(total, args.warmup * per_step))
     torch.cuda.synchronize()
     del wr
-  runs, done_runs = [], []
+  done_runs = []
   torch.cuda.synchronize()
   t0 = time.perf_counter()
-  for s in range(args.steps):
-    if not runs or runs[-1][1] >= total:
-      runs.append([_ragged_run(pa, specs, bij, args.seed + len(runs),
-                               adaptation, na, nr, tap,
-                               args.c3_group_adaptation,
-                               _class_splits(args)), 0])
-    run = runs[-1]
-    run[1] = min(total, run[1] + per_step)
-    run[0].advance_to(run[1])
-    if run[1] >= total:
-      done_runs.append((run[0].finish(), run[0].stats))
+  # the timed steps' transitions, run after run
+  timed = args.steps * per_step
+  left, k = timed, 0
+  while left > 0:
+    n = min(total, left)
+    run = _ragged_run(pa, specs, bij, args.seed + k, adaptation, na, nr, tap,
+                      args.c3_group_adaptation, _class_splits(args))
+    run.advance_to(n)
+    if n >= total:
+      done_runs.append((run.finish(), run.stats))
     torch.cuda.synchronize()
-    if (s + 1) % 5 == 0 or s + 1 == args.steps:
-      _say(f'ragged: step {s + 1}/{args.steps}')
+    left -= n
+    k += 1
+    _say(f'ragged: run {k} done, {timed - left} of {timed} timed '
+         'transitions')
   torch.cuda.synchronize()
   wall = time.perf_counter() - t0
   ms = 1000.0 * wall / max(args.steps, 1)
@@ -449,7 +457,8 @@ def ragged_glms_512(args, torch, pa):
                             num_burnin_steps=na, num_adaptation_steps=na,
                             step_size=_init_steps([specs[i]], adaptation),
                             target_accept_prob=tap,
-                            max_tree_depth=10, max_iters_per_launch=100,
+                            max_tree_depth=10,
+                            max_iters_per_launch=C3_ITERS_PER_LAUNCH,
                             adaptation=adaptation)
     torch.cuda.synchronize()
     ta = time.perf_counter()
