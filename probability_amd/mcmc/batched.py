@@ -64,10 +64,11 @@ class PosteriorBatch:
       multiple of it (else 1 is used).  Results depend on it only through
       the order of floating-point sums; equal values give bit-identical
       results alone or in any mix.
-    group_adaptation: use the chain groups in single-transition launches
-      (adaptation steps) too; by default only multi-transition launches
-      use them, since a single transition of a group lasts as long as its
-      longest trajectory.
+    group_adaptation: use the chain groups in the adaptation steps too; by
+      default only the sampling phase's launches use them, since a single
+      transition of a group lasts as long as its longest trajectory.  The
+      choice follows the phase, not a launch's length, so the draws do not
+      depend on max_iters_per_launch.
   """
 
   def __init__(self, specs, num_chains, bijectors=None, keys=None,
@@ -610,10 +611,11 @@ class _Run:
     a.unrolled_leapfrog_steps = self.unrolled
     a.target_kind = b.kind
     a.waves_per_chain = b.waves_per_chain
-    # chain groups in multi-transition launches, and in single transitions
-    # (adaptation steps, bound by each group's slowest chain) on request
-    a.chains_per_block = (b.chains_per_block if n > 1 or b.group_adaptation
-                          else 1)
+    # chain groups in the sampling phase, and in the adaptation steps (each
+    # bound by its groups' slowest chains) on request
+    sampling = t0 >= self.num_adapt
+    a.chains_per_block = (b.chains_per_block
+                          if sampling or b.group_adaptation else 1)
     a.max_energy_diff = float(self.max_energy_diff)
     a.problems = b.table.data_ptr()
     a.n_problems = b.n_problems

@@ -155,6 +155,27 @@ def test_sample_ragged_class_splits_equal_unsplit():
     assert float(res.step_size[k]) == float(base.step_size[k])
 
 
+@pytest.mark.parametrize('adaptation', ['dual_averaging', 'windowed'])
+def test_launch_length_leaves_draws_unchanged(adaptation):
+  # the sampling phase in launches of 3 transitions or in one launch, with
+  # the chain groups of the sampling launches: the same draws, trees, final
+  # states and step sizes (every draw is keyed by its iteration)
+  specs, bij = _ragged_problems(torch.float32)
+  chains = [4, 6, 5, 3, 7, 3]
+  kw = dict(num_results=7, num_burnin_steps=40, num_adaptation_steps=40,
+            step_size=0.05, target_accept_prob=0.8, seed=13, slot_dim=64,
+            adaptation=adaptation, chains_per_block={8: 2, 4: 2, 2: 2})
+  one = batched.sample_ragged(specs, chains, bijectors=bij,
+                              max_iters_per_launch=4096, **kw)
+  short = batched.sample_ragged(specs, chains, bijectors=bij,
+                                max_iters_per_launch=3, **kw)
+  for k in range(len(specs)):
+    for field in ('draws', 'leapfrogs_taken', 'final_state'):
+      np.testing.assert_array_equal(getattr(short, field)[k].cpu().numpy(),
+                                    getattr(one, field)[k].cpu().numpy())
+    assert float(short.step_size[k]) == float(one.step_size[k])
+
+
 def _glm_group_problems(dtype):
   r = np.random.default_rng(21)
   specs, bij, oracles = [], [], []
